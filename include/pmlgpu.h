@@ -1,0 +1,108 @@
+/*
+ * pmlgpu: polynomial-matrix multiplication over Z/pZ on one MI355X (gfx950).
+ *
+ * C ABI of the device library libpmlgpu.so.  Two layers:
+ *
+ *  1. The explicit status API (pmlg_*): packed, entry-major polynomial
+ *     matrices (entry (i,j) coefficient d of an r x c matrix of padded length
+ *     len at a[(i*c + j)*len + d], residues in [0,p)), with host-pointer and
+ *     device-pointer variants.  Every call returns a pmlg_status; the text of
+ *     the last failure is available from pmlg_last_error().
+ *
+ *  2. The FLINT-shaped drop-in (pmlgpu_flint.h): nmod_poly_mat_multiply,
+ *     nmod_poly_mat_mulmid, nmod_poly_mat_pmbasis, ... on nmod_poly_mat_t.
+ *
+ * Reference interfaces replaced (vneiger/pml, flint-extras/src):
+ *   pmlg_polmat_mul*     <- nmod_poly_mat_multiply    nmod_poly_mat_multiply.h:92
+ *                           (and the evaluation back-ends it dispatches to,
+ *                            nmod_poly_mat_mul_geometric :80, _vandermonde1/2 :63-64,
+ *                            _waksman :72, whose result is the same unique product)
+ *   pmlg_polmat_mulmid*  <- nmod_poly_mat_mulmid      nmod_poly_mat_multiply.h:122-124
+ *                           _nmod_poly_mat_mulmid     nmod_poly_mat_multiply.h:128-131
+ *   pmlg_pmbasis         <- nmod_poly_mat_pmbasis     nmod_poly_mat_approximant.h:282-285
+ *   pmlg_mbasis          <- nmod_poly_mat_mbasis      nmod_poly_mat_approximant.h:321-324
+ *   pmlg_polmat_mul_batch: no single reference counterpart; it runs many
+ *                          independent nmod_poly_mat_multiply calls at once.
+ *
+ * Supported moduli: primes p < 2^62 with 2^ceil(log2(len)) | p-1 for the
+ * transform length len the call needs (at most 2^15 for p >= 2^30, 2^16
+ * for p < 2^30).  Others return PMLG_ERR_UNSUPPORTED.
+ */
+#ifndef PMLGPU_H
+#define PMLGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PMLG_OK = 0,
+    PMLG_ERR_ARG = 1,          /* invalid argument (dimension, length, pointer) */
+    PMLG_ERR_UNSUPPORTED = 2,  /* modulus / length / shape outside what the device path handles */
+    PMLG_ERR_DEVICE = 3,       /* HIP runtime error */
+    PMLG_ERR_ALLOC = 4         /* device or host allocation failed */
+} pmlg_status;
+
+/* Pointwise-product kernel selection. */
+typedef enum {
+    PMLG_METHOD_AUTO = 0,  /* MFMA when the shape and modulus allow it, plain otherwise */
+    PMLG_METHOD_PLAIN = 1, /* per-lane exact 128-bit (or 64-bit) dot products */
+    PMLG_METHOD_MFMA = 2   /* int8-limb MFMA contraction with int32 accumulation */
+} pmlg_method;
+
+typedef struct pmlg_ctx_s *pmlg_ctx;
+
+/* A context holds one prime, its twiddle tables, a HIP stream and scratch
+ * buffers on one device.  Calls on one context are serialised internally. */
+pmlg_status pmlg_ctx_create(pmlg_ctx *ctx, uint64_t p, int device);
+void pmlg_ctx_destroy(pmlg_ctx ctx);
+/* log2 of the largest transform length usable with this prime */
+int pmlg_ctx_max_log_len(pmlg_ctx ctx);
+/* Stream used by the host-pointer calls (a hipStream_t). */
+void *pmlg_ctx_stream(pmlg_ctx ctx);
+pmlg_status pmlg_ctx_sync(pmlg_ctx ctx);
+
+/* Text describing the last failure on the calling thread ("" if none). */
+const char *pmlg_last_error(void);
+
+/* C = A * B.  A is m x k (padded length lenA), B is k x n (lenB), C is m x n
+ * with padded length lenA+lenB-1 (nothing is written when lenA or lenB is 0). */
+pmlg_status pmlg_polmat_mul(pmlg_ctx ctx, uint64_t *C, const uint64_t *A, const uint64_t *B,
+                            int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                            int method);
+/* Same, on device pointers, enqueued on `stream` (a hipStream_t, or NULL for
+ * the context's stream); returns without synchronising. */
+pmlg_status pmlg_polmat_mul_dev(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA,
+                                const uint64_t *dB, int64_t m, int64_t k, int64_t n,
+                                int64_t lenA, int64_t lenB, int method, void *stream);
+
+/* out = (A*B mod x^nhi) div x^nlo, padded length max(nhi-nlo, 0). */
+pmlg_status pmlg_polmat_mulmid(pmlg_ctx ctx, uint64_t *out, const uint64_t *A,
+                               const uint64_t *B, int64_t m, int64_t k, int64_t n,
+                               int64_t lenA, int64_t lenB, int64_t nlo, int64_t nhi,
+                               int method);
+pmlg_status pmlg_polmat_mulmid_dev(pmlg_ctx ctx, uint64_t *dout, const uint64_t *dA,
+                                   const uint64_t *dB, int64_t m, int64_t k, int64_t n,
+                                   int64_t lenA, int64_t lenB, int64_t nlo, int64_t nhi,
+                                   int method, void *stream);
+
+/* Per-stage device timing (HIP events around each launch).  Stages:
+ * 0 forward NTT, 1 pointwise product, 2 inverse NTT (+ merge). */
+void pmlg_profile_enable(pmlg_ctx ctx, int on);
+/* Synchronises, then returns the accumulated milliseconds and launch counts
+ * per stage (arrays of 3) and resets the accumulators. */
+pmlg_status pmlg_profile_read(pmlg_ctx ctx, double *ms, int64_t *launches);
+
+/* Device memory helpers (plain hipMalloc/hipMemcpy on the context's device). */
+pmlg_status pmlg_dev_alloc(pmlg_ctx ctx, void **ptr, uint64_t bytes);
+void pmlg_dev_free(pmlg_ctx ctx, void *ptr);
+pmlg_status pmlg_memcpy_h2d(pmlg_ctx ctx, void *dst, const void *src, uint64_t bytes);
+pmlg_status pmlg_memcpy_d2h(pmlg_ctx ctx, void *dst, const void *src, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMLGPU_H */

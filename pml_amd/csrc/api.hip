@@ -1,0 +1,585 @@
+// Host side of the device library: per-prime contexts, twiddle tables,
+// scratch buffers, stage timing and the extern "C" status API of pmlgpu.h.
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pmlgpu.h"
+#include "internal.hpp"
+#include "host_arith.hpp"
+
+using namespace pmlg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+pmlg_status fail(pmlg_status st, const std::string &msg)
+{
+    g_last_error = msg;
+    return st;
+}
+
+pmlg_status hip_fail(hipError_t e, const char *where)
+{
+    return fail(PMLG_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, where)                \
+    do {                                    \
+        hipError_t _e = (expr);             \
+        if (_e != hipSuccess)               \
+            return hip_fail(_e, where);     \
+    } while (0)
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes)
+            return hipSuccess;
+        if (ptr) {
+            // work enqueued earlier (possibly on a caller's stream) may still read it
+            (void)hipDeviceSynchronize();
+            (void)hipFree(ptr);
+        }
+        ptr = nullptr;
+        bytes = 0;
+        size_t want = b + b / 4;  // some headroom against regrowth
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        bytes = want;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+struct Tables {
+    void *mem = nullptr;
+    NttTab<uint64_t> t64{};
+    NttTab<uint32_t> t32{};
+};
+
+struct Prof {
+    int stage;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct pmlg_ctx_s {
+    uint64_t p = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ModConst mc{};
+    bool small = false;  // p < 2^30: 32-bit words on the device
+    int max_log = 0;     // log2 of the largest supported transform
+    uint64_t root = 0;   // primitive 2^max_log-th root of unity
+    std::vector<Tables> tabs;
+    DevBuf in_a, in_b, out_c, ev_a, ev_b, ev_c, tmp;
+    std::mutex mu;
+    bool profile = false;
+    std::vector<Prof> pending;
+    double ms[3] = {0, 0, 0};
+    int64_t launches[3] = {0, 0, 0};
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        if (prev != dev)
+            (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+    }
+};
+
+int ceil_log2(int64_t x)
+{
+    int l = 0;
+    while (((int64_t)1 << l) < x)
+        ++l;
+    return l;
+}
+
+// Builds (once) the root tables for transform length 2^logN.
+pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
+{
+    if ((int)ctx->tabs.size() <= logN)
+        ctx->tabs.resize(logN + 1);
+    Tables &T = ctx->tabs[logN];
+    if (!T.mem) {
+        const int64_t N = (int64_t)1 << logN;
+        const uint64_t p = ctx->p;
+        uint64_t w = host_powmod(ctx->root, (uint64_t)1 << (ctx->max_log - logN), p);
+        uint64_t wi = host_invmod(w, p);
+        const size_t wsz = ctx->small ? 4 : 8;
+        std::vector<unsigned char> h(4 * N * wsz);
+        uint64_t x = 1 % p, y = 1 % p;
+        for (int64_t t = 0; t < N; ++t) {
+            if (ctx->small) {
+                uint32_t *a = reinterpret_cast<uint32_t *>(h.data());
+                a[t] = (uint32_t)x;
+                a[N + t] = (uint32_t)host_shoup_pre32(x, p);
+                a[2 * N + t] = (uint32_t)y;
+                a[3 * N + t] = (uint32_t)host_shoup_pre32(y, p);
+            } else {
+                uint64_t *a = reinterpret_cast<uint64_t *>(h.data());
+                a[t] = x;
+                a[N + t] = host_shoup_pre64(x, p);
+                a[2 * N + t] = y;
+                a[3 * N + t] = host_shoup_pre64(y, p);
+            }
+            x = host_mulmod(x, w, p);
+            y = host_mulmod(y, wi, p);
+        }
+        void *mem = nullptr;
+        HIP_TRY(hipMalloc(&mem, h.size()), "hipMalloc(twiddles)");
+        hipError_t e = hipMemcpy(mem, h.data(), h.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(mem);
+            return hip_fail(e, "hipMemcpy(twiddles)");
+        }
+        T.mem = mem;
+        const uint64_t ninv = host_invmod((uint64_t)(N % p), p);
+        if (ctx->small) {
+            const uint32_t *a = reinterpret_cast<const uint32_t *>(mem);
+            T.t32 = NttTab<uint32_t>{a, a + N, a + 2 * N, a + 3 * N, (uint32_t)ninv,
+                                     (uint32_t)host_shoup_pre32(ninv, p)};
+        } else {
+            const uint64_t *a = reinterpret_cast<const uint64_t *>(mem);
+            T.t64 = NttTab<uint64_t>{a, a + N, a + 2 * N, a + 3 * N, ninv,
+                                     host_shoup_pre64(ninv, p)};
+        }
+    }
+    *out = &T;
+    return PMLG_OK;
+}
+
+void prof_begin(pmlg_ctx ctx, int stage, hipStream_t s, Prof *pr)
+{
+    pr->stage = -1;
+    if (!ctx->profile)
+        return;
+    if (hipEventCreate(&pr->a) != hipSuccess)
+        return;
+    if (hipEventCreate(&pr->b) != hipSuccess) {
+        (void)hipEventDestroy(pr->a);
+        return;
+    }
+    pr->stage = stage;
+    (void)hipEventRecord(pr->a, s);
+}
+
+void prof_end(pmlg_ctx ctx, hipStream_t s, Prof *pr)
+{
+    if (pr->stage < 0)
+        return;
+    (void)hipEventRecord(pr->b, s);
+    ctx->pending.push_back(*pr);
+}
+
+// Fill kernel (used instead of hipMemsetAsync so that the sequence stays
+// capturable in a HIP graph).
+__global__ void fill_zero_kernel(uint64_t *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+hipError_t fill_zero(uint64_t *p, int64_t n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+template <typename W>
+const NttTab<W> &tab_of(const Tables &T);
+template <>
+const NttTab<uint64_t> &tab_of<uint64_t>(const Tables &T)
+{
+    return T.t64;
+}
+template <>
+const NttTab<uint32_t> &tab_of<uint32_t>(const Tables &T)
+{
+    return T.t32;
+}
+
+// Evaluation/interpolation pipeline for coefficients [lo, hi) of A*B, with a
+// transform length 2^logN that is at least hi and at least lenC - lo.
+template <typename W>
+pmlg_status run_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const uint64_t *dA,
+                        const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t strideA,
+                        int64_t strideB, int64_t la, int64_t lb, int64_t lo, int64_t hi, int logN,
+                        PointwiseKind kind, hipStream_t s)
+{
+    const Tables *T = nullptr;
+    pmlg_status st = get_tables(ctx, logN, &T);
+    if (st != PMLG_OK)
+        return st;
+    const NttTab<W> &tab = tab_of<W>(*T);
+    const int64_t N = (int64_t)1 << logN;
+    const W p = (W)ctx->p;
+    HIP_TRY(ctx->ev_a.ensure((size_t)(m * k * N) * sizeof(W)), "hipMalloc(eval A)");
+    HIP_TRY(ctx->ev_b.ensure((size_t)(k * n * N) * sizeof(W)), "hipMalloc(eval B)");
+    HIP_TRY(ctx->ev_c.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(eval C)");
+    const int lim = NttLimits<W>::max_log_m;
+    W *tmp = nullptr;
+    if (logN > lim) {
+        HIP_TRY(ctx->tmp.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(tmp)");
+        tmp = static_cast<W *>(ctx->tmp.ptr);
+    }
+    W *Ah = static_cast<W *>(ctx->ev_a.ptr);
+    W *Bh = static_cast<W *>(ctx->ev_b.ptr);
+    W *Ch = static_cast<W *>(ctx->ev_c.ptr);
+
+    Prof pr;
+    prof_begin(ctx, 0, s, &pr);
+    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, tab, p, s)), "ntt_fwd(A)");
+    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, tab, p, s)), "ntt_fwd(B)");
+    prof_end(ctx, s, &pr);
+    prof_begin(ctx, 1, s, &pr);
+    HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, N, ctx->mc, s),
+            "pointwise");
+    prof_end(ctx, s, &pr);
+    prof_begin(ctx, 2, s, &pr);
+    HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, tab, p, tmp, s)),
+            "ntt_inv");
+    prof_end(ctx, s, &pr);
+    return PMLG_OK;
+}
+
+pmlg_status check_ctx(pmlg_ctx ctx)
+{
+    if (!ctx)
+        return fail(PMLG_ERR_ARG, "null context");
+    return PMLG_OK;
+}
+
+pmlg_status pick_kind(pmlg_ctx ctx, int method, int64_t m, int64_t k, int64_t n,
+                      PointwiseKind *kind)
+{
+    const bool mfma_ok = ctx->mc.limbs > 0 && mfma_supported((int)m, (int)k, (int)n, ctx->mc.limbs);
+    if (method == PMLG_METHOD_PLAIN) {
+        *kind = PW_PLAIN;
+    } else if (method == PMLG_METHOD_MFMA) {
+        if (!mfma_ok)
+            return fail(PMLG_ERR_UNSUPPORTED,
+                        "MFMA pointwise kernel: shape or modulus outside its limits");
+        *kind = PW_MFMA;
+    } else if (method == PMLG_METHOD_AUTO) {
+        *kind = mfma_ok ? PW_MFMA : PW_PLAIN;
+    } else {
+        return fail(PMLG_ERR_ARG, "unknown method");
+    }
+    return PMLG_OK;
+}
+
+// Core of every product call: coefficients [nlo, nhi) of A*B into dOut
+// (padded length nhi - nlo), all pointers on the device.
+pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
+                              const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t lenA,
+                              int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s)
+{
+    if (m < 0 || k < 0 || n < 0 || lenA < 0 || lenB < 0 || nlo < 0)
+        return fail(PMLG_ERR_ARG, "negative dimension, length or nlo");
+    if (m > INT32_MAX / 64 || k > INT32_MAX / 64 || n > INT32_MAX / 64)
+        return fail(PMLG_ERR_ARG, "dimension too large");
+    const int64_t width = nhi - nlo;
+    if (width <= 0 || m == 0 || n == 0)
+        return PMLG_OK;
+    if ((!dOut) || ((k > 0 && lenA > 0 && lenB > 0) && (!dA || !dB)))
+        return fail(PMLG_ERR_ARG, "null buffer");
+    const int64_t lenC = (lenA > 0 && lenB > 0) ? lenA + lenB - 1 : 0;
+    const int64_t hi_eff = nhi < lenC ? nhi : lenC;
+    if (k == 0 || hi_eff <= nlo) {
+        HIP_TRY(fill_zero(dOut, m * n * width, s), "fill_zero");
+        return PMLG_OK;
+    }
+    // Only coefficients below hi_eff of each operand matter; they are read in
+    // place with their original strides, so clip the lengths.
+    const int64_t la = lenA < hi_eff ? lenA : hi_eff;
+    const int64_t lb = lenB < hi_eff ? lenB : hi_eff;
+    const int64_t lc = la + lb - 1;
+    // cyclic length: no wrapped term may land in [nlo, hi_eff)
+    int64_t need = hi_eff > lc - nlo ? hi_eff : lc - nlo;
+    const int logN = ceil_log2(need);
+    if (logN > ctx->max_log)
+        return fail(PMLG_ERR_UNSUPPORTED,
+                    "transform length 2^" + std::to_string(logN) +
+                        " not supported by this prime (2-adic valuation or size limit)");
+    PointwiseKind kind;
+    pmlg_status st = pick_kind(ctx, method, m, k, n, &kind);
+    if (st != PMLG_OK)
+        return st;
+    if (hi_eff < nhi)  // coefficients beyond the product length are zero
+        HIP_TRY(fill_zero(dOut, m * n * width, s), "fill_zero");
+    // operands keep their strides lenA / lenB; la/lb only bound what is read
+    if (ctx->small)
+        return run_product<uint32_t>(ctx, dOut, width, dA, dB, m, k, n, lenA, lenB, la, lb, nlo,
+                                     hi_eff, logN, kind, s);
+    return run_product<uint64_t>(ctx, dOut, width, dA, dB, m, k, n, lenA, lenB, la, lb, nlo,
+                                 hi_eff, logN, kind, s);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// extern "C" API
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *pmlg_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+pmlg_status pmlg_ctx_create(pmlg_ctx *out, uint64_t p, int device)
+{
+    if (!out)
+        return fail(PMLG_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (p < 3 || p >= ((uint64_t)1 << 62) || !host_is_prime(p))
+        return fail(PMLG_ERR_UNSUPPORTED, "modulus must be an odd prime below 2^62");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(PMLG_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev)
+        return fail(PMLG_ERR_ARG, "device index out of range");
+    pmlg_ctx ctx = new (std::nothrow) pmlg_ctx_s();
+    if (!ctx)
+        return fail(PMLG_ERR_ALLOC, "out of host memory");
+    ctx->p = p;
+    ctx->device = device;
+    ctx->small = p < ((uint64_t)1 << 30);
+    int v2 = __builtin_ctzll(p - 1);
+    const int cap = ctx->small ? NttLimits<uint32_t>::max_log_m + 2
+                               : NttLimits<uint64_t>::max_log_m + 2;
+    ctx->max_log = v2 < cap ? v2 : cap;
+    ctx->root = host_root_of_unity_2k(p, ctx->max_log);
+    ctx->mc = host_modconst(p);
+    DeviceGuard g(device);
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    *out = ctx;
+    return PMLG_OK;
+}
+
+void pmlg_ctx_destroy(pmlg_ctx ctx)
+{
+    if (!ctx)
+        return;
+    {
+        DeviceGuard g(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &pr : ctx->pending) {
+            (void)hipEventDestroy(pr.a);
+            (void)hipEventDestroy(pr.b);
+        }
+        for (auto &t : ctx->tabs)
+            if (t.mem)
+                (void)hipFree(t.mem);
+        ctx->in_a.release();
+        ctx->in_b.release();
+        ctx->out_c.release();
+        ctx->ev_a.release();
+        ctx->ev_b.release();
+        ctx->ev_c.release();
+        ctx->tmp.release();
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int pmlg_ctx_max_log_len(pmlg_ctx ctx)
+{
+    return ctx ? ctx->max_log : -1;
+}
+
+void *pmlg_ctx_stream(pmlg_ctx ctx)
+{
+    return ctx ? (void *)ctx->stream : nullptr;
+}
+
+pmlg_status pmlg_ctx_sync(pmlg_ctx ctx)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    DeviceGuard g(ctx->device);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_polmat_mulmid_dev(pmlg_ctx ctx, uint64_t *dout, const uint64_t *dA,
+                                   const uint64_t *dB, int64_t m, int64_t k, int64_t n,
+                                   int64_t lenA, int64_t lenB, int64_t nlo, int64_t nhi,
+                                   int method, void *stream)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    return mulmid_dev_locked(ctx, dout, dA, dB, m, k, n, lenA, lenB, nlo, nhi, method, s);
+}
+
+pmlg_status pmlg_polmat_mul_dev(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA,
+                                const uint64_t *dB, int64_t m, int64_t k, int64_t n,
+                                int64_t lenA, int64_t lenB, int method, void *stream)
+{
+    if (lenA <= 0 || lenB <= 0)
+        return lenA < 0 || lenB < 0 ? fail(PMLG_ERR_ARG, "negative length") : PMLG_OK;
+    return pmlg_polmat_mulmid_dev(ctx, dC, dA, dB, m, k, n, lenA, lenB, 0, lenA + lenB - 1,
+                                  method, stream);
+}
+
+pmlg_status pmlg_polmat_mulmid(pmlg_ctx ctx, uint64_t *out, const uint64_t *A, const uint64_t *B,
+                               int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                               int64_t nlo, int64_t nhi, int method)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (m < 0 || k < 0 || n < 0 || lenA < 0 || lenB < 0 || nlo < 0)
+        return fail(PMLG_ERR_ARG, "negative dimension, length or nlo");
+    const int64_t width = nhi - nlo;
+    if (width <= 0 || m == 0 || n == 0)
+        return PMLG_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t ba = (size_t)(m * k * lenA) * 8, bb = (size_t)(k * n * lenB) * 8;
+    const size_t bc = (size_t)(m * n * width) * 8;
+    HIP_TRY(ctx->in_a.ensure(ba ? ba : 8), "hipMalloc(A)");
+    HIP_TRY(ctx->in_b.ensure(bb ? bb : 8), "hipMalloc(B)");
+    HIP_TRY(ctx->out_c.ensure(bc), "hipMalloc(C)");
+    if (ba)
+        HIP_TRY(hipMemcpyAsync(ctx->in_a.ptr, A, ba, hipMemcpyHostToDevice, s), "H2D(A)");
+    if (bb)
+        HIP_TRY(hipMemcpyAsync(ctx->in_b.ptr, B, bb, hipMemcpyHostToDevice, s), "H2D(B)");
+    st = mulmid_dev_locked(ctx, static_cast<uint64_t *>(ctx->out_c.ptr),
+                           static_cast<const uint64_t *>(ctx->in_a.ptr),
+                           static_cast<const uint64_t *>(ctx->in_b.ptr), m, k, n, lenA, lenB, nlo,
+                           nhi, method, s);
+    if (st != PMLG_OK) {
+        (void)hipStreamSynchronize(s);
+        return st;
+    }
+    HIP_TRY(hipMemcpyAsync(out, ctx->out_c.ptr, bc, hipMemcpyDeviceToHost, s), "D2H(C)");
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_polmat_mul(pmlg_ctx ctx, uint64_t *C, const uint64_t *A, const uint64_t *B,
+                            int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                            int method)
+{
+    if (lenA <= 0 || lenB <= 0)
+        return lenA < 0 || lenB < 0 ? fail(PMLG_ERR_ARG, "negative length") : PMLG_OK;
+    return pmlg_polmat_mulmid(ctx, C, A, B, m, k, n, lenA, lenB, 0, lenA + lenB - 1, method);
+}
+
+void pmlg_profile_enable(pmlg_ctx ctx, int on)
+{
+    if (ctx)
+        ctx->profile = on != 0;
+}
+
+pmlg_status pmlg_profile_read(pmlg_ctx ctx, double *ms, int64_t *launches)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    for (auto &pr : ctx->pending) {
+        float t = 0.f;
+        if (hipEventSynchronize(pr.b) == hipSuccess &&
+            hipEventElapsedTime(&t, pr.a, pr.b) == hipSuccess) {
+            ctx->ms[pr.stage] += t;
+            ctx->launches[pr.stage] += 1;
+        }
+        (void)hipEventDestroy(pr.a);
+        (void)hipEventDestroy(pr.b);
+    }
+    ctx->pending.clear();
+    for (int i = 0; i < 3; ++i) {
+        if (ms)
+            ms[i] = ctx->ms[i];
+        if (launches)
+            launches[i] = ctx->launches[i];
+        ctx->ms[i] = 0;
+        ctx->launches[i] = 0;
+    }
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_dev_alloc(pmlg_ctx ctx, void **ptr, uint64_t bytes)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    DeviceGuard g(ctx->device);
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 8), "hipMalloc");
+    return PMLG_OK;
+}
+
+void pmlg_dev_free(pmlg_ctx ctx, void *ptr)
+{
+    if (!ctx || !ptr)
+        return;
+    DeviceGuard g(ctx->device);
+    (void)hipFree(ptr);
+}
+
+pmlg_status pmlg_memcpy_h2d(pmlg_ctx ctx, void *dst, const void *src, uint64_t bytes)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    DeviceGuard g(ctx->device);
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_memcpy_d2h(pmlg_ctx ctx, void *dst, const void *src, uint64_t bytes)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    DeviceGuard g(ctx->device);
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return PMLG_OK;
+}
+
+}  // extern "C"
