@@ -261,6 +261,8 @@ pmlg_status run_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
     Prof pr;
     prof_begin(ctx, 0, s, &pr);
     HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, tab, p, s)), "ntt_fwd(A)");
+    prof_end(ctx, s, &pr);
+    prof_begin(ctx, 0, s, &pr);
     HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, tab, p, s)), "ntt_fwd(B)");
     prof_end(ctx, s, &pr);
     prof_begin(ctx, 1, s, &pr);
