@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (reference "
+                         "kernel for calibrating hardware byte counters)")
     return ap.parse_args()
 
 
@@ -110,6 +113,14 @@ def main():
     ctx.mul_dev(dC.ptr, dA.ptr, dB.ptr, M, K, N_, LEN, LEN, method)
     ctx.sync()
     dC.download(C)
+
+    if args.copy_mib:
+        nb = args.copy_mib << 20
+        src, dst = pml_amd.DeviceArray(ctx, nb), pml_amd.DeviceArray(ctx, nb)
+        ctx.dev_copy(dst.ptr, src.ptr, nb)
+        ctx.sync()
+        src.free()
+        dst.free()
 
     st = stage_times(ctx, dC, dA, dB, method, 20)
     # dominant kernel: largest total device time per product

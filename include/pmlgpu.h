@@ -95,6 +95,11 @@ void pmlg_profile_enable(pmlg_ctx ctx, int on);
  * per stage (arrays of 3) and resets the accumulators. */
 pmlg_status pmlg_profile_read(pmlg_ctx ctx, double *ms, int64_t *launches);
 
+/* Plain device-to-device copy kernel (16 bytes per lane; pointers and size
+ * 16-byte aligned), enqueued on `stream` or the context's stream.  Used as the
+ * reference kernel when hardware byte counters are calibrated. */
+pmlg_status pmlg_dev_copy(pmlg_ctx ctx, void *dst, const void *src, uint64_t bytes, void *stream);
+
 /* Device memory helpers (plain hipMalloc/hipMemcpy on the context's device). */
 pmlg_status pmlg_dev_alloc(pmlg_ctx ctx, void **ptr, uint64_t bytes);
 void pmlg_dev_free(pmlg_ctx ctx, void *ptr);

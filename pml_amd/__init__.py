@@ -75,6 +75,8 @@ def lib():
         L.pmlg_profile_enable.argtypes = [vp, c_int]
         L.pmlg_profile_read.restype = st
         L.pmlg_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), _i64p]
+        L.pmlg_dev_copy.restype = st
+        L.pmlg_dev_copy.argtypes = [vp, vp, vp, u64, vp]
         L.pmlg_dev_alloc.restype = st
         L.pmlg_dev_alloc.argtypes = [vp, ctypes.POINTER(vp), u64]
         L.pmlg_dev_free.restype = None
@@ -208,6 +210,9 @@ class Context:
             _check(lib().pmlg_polmat_mulmid(self.handle, _p(out), _p(A), _p(B), m, k, n, la, lb,
                                             nlo, nhi, _method(method)))
         return out
+
+    def dev_copy(self, dst, src, nbytes, stream=None):
+        _check(lib().pmlg_dev_copy(self.handle, int(dst), int(src), int(nbytes), stream))
 
     def mul_dev(self, dC, dA, dB, m, k, n, la, lb, method="auto", stream=None):
         """Device-pointer product, enqueued without synchronisation."""

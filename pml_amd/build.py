@@ -21,6 +21,9 @@ HIP_SOURCES = ["ntt.hip", "pointwise.hip", "api.hip", "pmbasis.hip", "batch.hip"
 C_SOURCES = ["dropin.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# per-file extra flags: keep MFMA results in VGPRs (no AGPR round trips before the fold)
+HIP_EXTRA = {"pointwise.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -47,7 +50,7 @@ def build(verbose=False):
         if not os.path.exists(path):
             continue
         obj = os.path.join(BUILD, src + ".o")
-        jobs.append([HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj])
+        jobs.append([HIPCC] + HIP_FLAGS + HIP_EXTRA.get(src, []) + ["-c", path, "-o", obj])
     for src in C_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):

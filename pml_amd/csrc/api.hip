@@ -1,5 +1,7 @@
 // Host side of the device library: per-prime contexts, twiddle tables,
 // scratch buffers, stage timing and the extern "C" status API of pmlgpu.h.
+#include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -88,6 +90,7 @@ struct pmlg_ctx_s {
     std::vector<Tables> tabs;
     DevBuf in_a, in_b, out_c, ev_a, ev_b, ev_c, tmp;
     std::mutex mu;
+    NttTune tune;
     bool profile = false;
     std::vector<Prof> pending;
     double ms[3] = {0, 0, 0};
@@ -132,7 +135,12 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
         uint64_t w = host_powmod(ctx->root, (uint64_t)1 << (ctx->max_log - logN), p);
         uint64_t wi = host_invmod(w, p);
         const size_t wsz = ctx->small ? 4 : 8;
-        std::vector<unsigned char> h(4 * N * wsz);
+        // full tables (4N words) followed by the per-stage tables of the
+        // LDS sub-transform (2M words forward, 2M words inverse)
+        const int lim = ctx->small ? NttLimits<uint32_t>::max_log_m : NttLimits<uint64_t>::max_log_m;
+        const int logM = logN < lim ? logN : lim;
+        const int64_t M = (int64_t)1 << logM;
+        std::vector<unsigned char> h((4 * N + 4 * M) * wsz);
         uint64_t x = 1 % p, y = 1 % p;
         for (int64_t t = 0; t < N; ++t) {
             if (ctx->small) {
@@ -151,6 +159,37 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
             x = host_mulmod(x, w, p);
             y = host_mulmod(y, wi, p);
         }
+        {
+            // sub-transform root w_M = w^(N/M); stage s entry x = w_M^(x 2^s)
+            const uint64_t wm = host_powmod(w, (uint64_t)(N / M), p);
+            const uint64_t wmi = host_invmod(wm, p);
+            for (int s = 0; s < logM; ++s) {
+                const int64_t off = M - ((int64_t)1 << (logM - s));
+                const int64_t cnt = (int64_t)1 << (logM - 1 - s);
+                const uint64_t step = host_powmod(wm, (uint64_t)1 << s, p);
+                const uint64_t stepi = host_powmod(wmi, (uint64_t)1 << s, p);
+                uint64_t a = 1 % p, b = 1 % p;
+                for (int64_t x = 0; x < cnt; ++x) {
+                    const int64_t f = 4 * N + 2 * (off + x);  // forward pair index (words)
+                    const int64_t g = 4 * N + 2 * M + 2 * (off + x);
+                    if (ctx->small) {
+                        uint32_t *d = reinterpret_cast<uint32_t *>(h.data());
+                        d[f] = (uint32_t)a;
+                        d[f + 1] = (uint32_t)host_shoup_pre32(a, p);
+                        d[g] = (uint32_t)b;
+                        d[g + 1] = (uint32_t)host_shoup_pre32(b, p);
+                    } else {
+                        uint64_t *d = reinterpret_cast<uint64_t *>(h.data());
+                        d[f] = a;
+                        d[f + 1] = host_shoup_pre64(a, p);
+                        d[g] = b;
+                        d[g + 1] = host_shoup_pre64(b, p);
+                    }
+                    a = host_mulmod(a, step, p);
+                    b = host_mulmod(b, stepi, p);
+                }
+            }
+        }
         void *mem = nullptr;
         HIP_TRY(hipMalloc(&mem, h.size()), "hipMalloc(twiddles)");
         hipError_t e = hipMemcpy(mem, h.data(), h.size(), hipMemcpyHostToDevice);
@@ -160,14 +199,18 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
         }
         T.mem = mem;
         const uint64_t ninv = host_invmod((uint64_t)(N % p), p);
+        const uint64_t ninvr = host_mulmod(ninv, ctx->mc.r64, p);
         if (ctx->small) {
             const uint32_t *a = reinterpret_cast<const uint32_t *>(mem);
             T.t32 = NttTab<uint32_t>{a, a + N, a + 2 * N, a + 3 * N, (uint32_t)ninv,
-                                     (uint32_t)host_shoup_pre32(ninv, p)};
+                                     (uint32_t)host_shoup_pre32(ninv, p), a + 4 * N,
+                                     a + 4 * N + 2 * M, (uint32_t)ninvr,
+                                     (uint32_t)host_shoup_pre32(ninvr, p)};
         } else {
             const uint64_t *a = reinterpret_cast<const uint64_t *>(mem);
             T.t64 = NttTab<uint64_t>{a, a + N, a + 2 * N, a + 3 * N, ninv,
-                                     host_shoup_pre64(ninv, p)};
+                                     host_shoup_pre64(ninv, p), a + 4 * N, a + 4 * N + 2 * M,
+                                     ninvr, host_shoup_pre64(ninvr, p)};
         }
     }
     *out = &T;
@@ -204,6 +247,14 @@ __global__ void fill_zero_kernel(uint64_t *__restrict__ p, int64_t n)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0;
+}
+
+// Plain 16-byte-per-lane device copy: the reference point for byte counters.
+__global__ void copy16_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, int64_t n16)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 hipError_t fill_zero(uint64_t *p, int64_t n, hipStream_t s)
@@ -258,19 +309,35 @@ pmlg_status run_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
     W *Bh = static_cast<W *>(ctx->ev_b.ptr);
     W *Ch = static_cast<W *>(ctx->ev_c.ptr);
 
+    // point-blocked evaluation layout: 32-byte runs per entry
+    const int logP = std::min(sizeof(W) == 8 ? 2 : 3, logN);
+    const NttTune &tune = ctx->tune;
     Prof pr;
     prof_begin(ctx, 0, s, &pr);
-    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, tab, p, s)), "ntt_fwd(A)");
+    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, logP, tune, tab, p, s)),
+            "ntt_fwd(A)");
     prof_end(ctx, s, &pr);
     prof_begin(ctx, 0, s, &pr);
-    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, tab, p, s)), "ntt_fwd(B)");
+    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, logP, tune, tab, p, s)),
+            "ntt_fwd(B)");
     prof_end(ctx, s, &pr);
+    // The 8-limb MFMA product leaves V 2^-64 mod p (Montgomery REDC) in [0, 4p);
+    // the inverse transform then scales by N^-1 2^64 instead of N^-1.
+    ModConst mc = ctx->mc;
+    NttTab<W> itab = tab;
+    if (kind == PW_MFMA && sizeof(W) == 8) {
+        const unsigned __int128 vmax = (unsigned __int128)k * (ctx->p - 1) * (ctx->p - 1);
+        mc.redc_small = vmax < (((unsigned __int128)(3 * ctx->p - 1)) << 64) ? 1 : 0;
+        itab.ninv = tab.ninvr;
+        itab.ninvq = tab.ninvrq;
+    }
     prof_begin(ctx, 1, s, &pr);
-    HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, N, ctx->mc, s),
+    HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, N, logP, mc, s),
             "pointwise");
     prof_end(ctx, s, &pr);
     prof_begin(ctx, 2, s, &pr);
-    HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, tab, p, tmp, s)),
+    HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, logP, tune,
+                                         itab, p, tmp, s)),
             "ntt_inv");
     prof_end(ctx, s, &pr);
     return PMLG_OK;
@@ -286,7 +353,7 @@ pmlg_status check_ctx(pmlg_ctx ctx)
 pmlg_status pick_kind(pmlg_ctx ctx, int method, int64_t m, int64_t k, int64_t n,
                       PointwiseKind *kind)
 {
-    const bool mfma_ok = ctx->mc.limbs > 0 && mfma_supported((int)m, (int)k, (int)n, ctx->mc.limbs);
+    const bool mfma_ok = ctx->mc.limbs > 0 && mfma_supported((int)m, (int)k, (int)n, ctx->mc.limbs, ctx->p);
     if (method == PMLG_METHOD_PLAIN) {
         *kind = PW_PLAIN;
     } else if (method == PMLG_METHOD_MFMA) {
@@ -385,6 +452,11 @@ pmlg_status pmlg_ctx_create(pmlg_ctx *out, uint64_t p, int device)
     ctx->max_log = v2 < cap ? v2 : cap;
     ctx->root = host_root_of_unity_2k(p, ctx->max_log);
     ctx->mc = host_modconst(p);
+    // radix of the register transforms (autotuned values may replace these)
+    if (const char *v = std::getenv("PMLG_NTT_FWD_LOGE"))
+        ctx->tune.fwd_log_e = std::atoi(v) == 4 ? 4 : 5;
+    if (const char *v = std::getenv("PMLG_NTT_INV_LOGE"))
+        ctx->tune.inv_log_e = std::atoi(v) == 5 ? 5 : 4;
     DeviceGuard g(device);
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -543,6 +615,26 @@ pmlg_status pmlg_profile_read(pmlg_ctx ctx, double *ms, int64_t *launches)
         ctx->ms[i] = 0;
         ctx->launches[i] = 0;
     }
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_dev_copy(pmlg_ctx ctx, void *dst, const void *src, uint64_t bytes, void *stream)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(PMLG_ERR_ARG, "pmlg_dev_copy needs 16-byte aligned pointers and size");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const int64_t n16 = (int64_t)(bytes / 16);
+    int64_t blocks = (n16 + 255) / 256;
+    if (blocks > 16384)
+        blocks = 16384;
+    if (n16)
+        hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                           static_cast<uint4 *>(dst), static_cast<const uint4 *>(src), n16);
+    HIP_TRY(hipGetLastError(), "copy16_kernel");
     return PMLG_OK;
 }
 

@@ -115,6 +115,16 @@ static inline ModConst host_modconst(uint64_t p)
     mc.c80q = host_shoup_pre64(mc.c80, p);
     mc.c32 = host_pow2_mod(32, p);
     mc.c32q = host_shoup_pre64(mc.c32, p);
+    {
+        // -p^-1 mod 2^64 by Newton iteration (p odd)
+        uint64_t inv = p;  // correct to 3 bits
+        for (int i = 0; i < 6; ++i)
+            inv *= 2 - p * inv;
+        mc.pinvneg = (uint64_t)0 - inv;
+    }
+    mc.r64 = mc.c64;
+    mc.r64q = mc.c64q;
+    mc.redc_small = 0;
     const uint64_t two61 = (uint64_t)1 << 61;
     mc.goff = p * ((two61 + p - 1) / p);
     int bits = 0;

@@ -108,6 +108,9 @@ struct ModConst {
     uint64_t c80, c80q; // 2^80 mod p
     uint64_t c32, c32q; // 2^32 mod p
     uint64_t goff;      // multiple of p, >= 2^61 (offset making signed partial sums nonnegative)
+    uint64_t pinvneg;   // -p^-1 mod 2^64 (Montgomery REDC)
+    uint64_t r64, r64q; // 2^64 mod p = Montgomery factor R, Shoup companion
+    int redc_small;     // set per call: REDC output already below 4p
     uint32_t p32, p32x2, p32q_unused;
     int bits;           // bit length of p
     int limbs;          // 8 for p < 2^62, 4 for p < 2^30

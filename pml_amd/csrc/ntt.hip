@@ -18,6 +18,23 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Point-blocked evaluation layout: position `pos` of entry `e` (of `count`
+// entries) lives at ((pos >> logP) * count + e) * P + (pos mod P), P = 2^logP,
+// so that the P values of every entry at P consecutive points form one 32-byte
+// run and all entries of one point block are contiguous.
+__device__ __forceinline__ int64_t blk_addr(int64_t e, int64_t pos, int64_t count, int logP)
+{
+    return (((pos >> logP) * count + e) << logP) | (pos & ((1 << logP) - 1));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give the workgroups
+// of one XCD consecutive units so that neighbouring entries (which share
+// 128-byte lines in the blocked layout) meet in the same L2.
+__device__ __forceinline__ int64_t xcd_unit(int64_t b, int64_t units)
+{
+    return (units & 7) ? b : (b & 7) * (units >> 3) + (b >> 3);
+}
+
 template <typename W>
 __device__ __forceinline__ W tw_mul(W x, const W *w, const W *wq, int64_t idx, W p)
 {
@@ -27,7 +44,7 @@ __device__ __forceinline__ W tw_mul(W x, const W *w, const W *wq, int64_t idx, W
 template <typename Win, typename W>
 __global__ void __launch_bounds__(kThreads)
 ntt_fwd_kernel(const Win *__restrict__ in, int64_t in_stride, int64_t in_len,
-               W *__restrict__ out, int64_t units, int logN, int logR, int lognb,
+               W *__restrict__ out, int64_t units, int logN, int logR, int lognb, int logP,
                NttTab<W> tab, W p)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -94,7 +111,7 @@ ntt_fwd_kernel(const Win *__restrict__ in, int64_t in_stride, int64_t in_len,
         if (u < units) {
             const int64_t e = u >> logR;
             const int g = (int)(u & (R - 1));
-            out[e * N + (int64_t)g * M + i1] = reduce_2p<W>(s[idx], p);
+            out[blk_addr(e, (int64_t)g * M + i1, units >> logR, logP)] = reduce_2p<W>(s[idx], p);
         }
     }
 }
@@ -103,7 +120,7 @@ template <typename W, typename Wout>
 __global__ void __launch_bounds__(kThreads)
 ntt_inv_kernel(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
                int64_t hi, W *__restrict__ tmp, int64_t units, int logN, int logR, int lognb,
-               NttTab<W> tab, W p)
+               int logP, NttTab<W> tab, W p)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *s = reinterpret_cast<W *>(smem_raw);
@@ -123,7 +140,8 @@ ntt_inv_kernel(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_str
         if (u < units) {
             const int64_t e = u >> logR;
             const int g = (int)(u & (R - 1));
-            v = in[e * N + (int64_t)g * M + i1];
+            v = in[blk_addr(e, (int64_t)g * M + i1, units >> logR, logP)];
+            v = v >= p2 ? v - p2 : v;  // products may arrive in [0, 4p)
         }
         s[idx] = v;
     }
@@ -200,31 +218,521 @@ inline int pick_lognb(int logM)
     return logM >= want ? 0 : want - logM;
 }
 
+// ===========================================================================
+// Register-resident transforms for 2^11 <= M <= 2^14.
+//
+// Each of the T = M / E threads keeps E = 2^LOGE elements in registers.  The
+// L = log2 M radix-2 stages are grouped into passes of LOGE stages (the last
+// one possibly shorter); a pass starting at stage S0 with R stages acts on the
+// index bits [BLO, BLO+R), BLO = L-S0-R, and every thread owns whole groups of
+// 2^R elements that differ only in those bits.  Between passes the elements go
+// through LDS, laid out with one padding word every 32 to spread the banks.
+// Stage s pairs bit b = L-1-s; the twiddle of a pair whose lower index is i is
+// w_M^((i mod 2^b) 2^s), read from the per-stage table at x = i mod 2^b.
+// ===========================================================================
+
+__device__ __forceinline__ int swz(int a)
+{
+    return a + (a >> 5);
+}
+
+template <typename W>
+__device__ __forceinline__ void load_tw(const W *__restrict__ tab, int idx, W &w, W &wq)
+{
+    if constexpr (sizeof(W) == 8) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(tab + 2 * idx);
+        w = ((uint64_t)v.y << 32) | v.x;
+        wq = ((uint64_t)v.w << 32) | v.z;
+    } else {
+        const uint2 v = *reinterpret_cast<const uint2 *>(tab + 2 * idx);
+        w = v.x;
+        wq = v.y;
+    }
+}
+
+// Butterfly constants.  Strict mode keeps values in [0, 2p) (any p < 2^62);
+// loose mode (64-bit words, p < 2^61) keeps them in [0, 4p) and uses a
+// cheaper Shoup product whose quotient may be short by at most 2.
+template <typename W>
+struct Bfly {
+    W p, p2, p4, np, np4;  // np = -p mod 2^64, np4 = -4p mod 2^64
+    __device__ explicit Bfly(W pp) : p(pp), p2(pp + pp), p4(4 * pp), np((W)0 - pp), np4((W)0 - 4 * pp) {}
+};
+
+// x*w mod p in [0, 4p) for any 64-bit x: q' = xh*qh + hi(xl*qh) + hi(xh*ql)
+// is the Shoup quotient minus 0, 1 or 2, and x*w - q'*p = x*w + q'*(-p) mod 2^64.
+__device__ __forceinline__ uint64_t shoup4(uint64_t x, uint64_t w, uint64_t wq, uint64_t np)
+{
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
+    const uint32_t ql = (uint32_t)wq, qh = (uint32_t)(wq >> 32);
+    uint64_t q = (uint64_t)xh * qh + __umulhi(xl, qh);
+    q += __umulhi(xh, ql);
+    const uint32_t q0 = (uint32_t)q, q1 = (uint32_t)(q >> 32);
+    const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
+    const uint32_t n0 = (uint32_t)np, n1 = (uint32_t)(np >> 32);
+    uint64_t r = (uint64_t)xl * w0;
+    r += (uint64_t)q0 * n0;
+    const uint32_t hi = (uint32_t)(r >> 32) + xl * w1 + xh * w0 + q0 * n1 + q1 * n0;
+    return ((uint64_t)hi << 32) | (uint32_t)r;
+}
+
+template <typename W, bool LOOSE>
+__device__ __forceinline__ void bfly_dif(W &a, W &c, W w, W wq, const Bfly<W> &k)
+{
+    if constexpr (LOOSE) {
+        W s = a + c;
+        s = s >= k.p4 ? s + k.np4 : s;
+        const W d = a + k.p4 - c;
+        a = s;
+        c = shoup4(d, w, wq, k.np);
+    } else {
+        const W u = a, v = c;
+        a = add_2p<W>(u, v, k.p2);
+        c = shoup_lazy(sub_2p<W>(u, v, k.p2), w, wq, k.p);
+    }
+}
+
+template <typename W, bool LOOSE>
+__device__ __forceinline__ void bfly_dit(W &a, W &c, W w, W wq, const Bfly<W> &k)
+{
+    if constexpr (LOOSE) {
+        const W v = shoup4(c, w, wq, k.np);
+        W s = a + v;
+        s = s >= k.p4 ? s + k.np4 : s;
+        W d = a + k.p4 - v;
+        d = d >= k.p4 ? d + k.np4 : d;
+        a = s;
+        c = d;
+    } else {
+        const W u = a;
+        const W v = shoup_lazy(c, w, wq, k.p);
+        a = add_2p<W>(u, v, k.p2);
+        const W d = u + k.p2 - v;
+        c = d >= k.p2 ? d - k.p2 : d;
+    }
+}
+
+// [0, 4p) (loose) or [0, 2p) (strict) -> [0, p)
+template <typename W, bool LOOSE>
+__device__ __forceinline__ W bfly_final(W x, const Bfly<W> &k)
+{
+    if constexpr (LOOSE)
+        x = x >= k.p2 ? x - k.p2 : x;
+    return x >= k.p ? x - k.p : x;
+}
+
+template <int L, int LOGE, int S0>
+struct PassShape {
+    static constexpr int R = (L - S0 < LOGE) ? (L - S0) : LOGE;
+    static constexpr int G = 1 << (LOGE - R);  // groups per thread
+    static constexpr int T = 1 << (L - LOGE);  // threads
+    static constexpr int BLO = L - S0 - R;
+    __device__ static __forceinline__ int index(int t, int g, int j)
+    {
+        const int gi = t + T * g;
+        const int glo = gi & ((1 << BLO) - 1);
+        const int ghi = gi >> BLO;
+        return (ghi << (BLO + R)) | (j << BLO) | glo;
+    }
+};
+
+template <typename W, int L, int LOGE, int S0>
+__device__ __forceinline__ void lds_store(const W (&x)[1 << LOGE], W *lds, int t)
+{
+    using PS = PassShape<L, LOGE, S0>;
+#pragma unroll
+    for (int g = 0; g < PS::G; ++g)
+#pragma unroll
+        for (int j = 0; j < (1 << PS::R); ++j)
+            lds[swz(PS::index(t, g, j))] = x[g * (1 << PS::R) + j];
+}
+
+template <typename W, int L, int LOGE, int S0>
+__device__ __forceinline__ void lds_load(W (&x)[1 << LOGE], const W *lds, int t)
+{
+    using PS = PassShape<L, LOGE, S0>;
+#pragma unroll
+    for (int g = 0; g < PS::G; ++g)
+#pragma unroll
+        for (int j = 0; j < (1 << PS::R); ++j)
+            x[g * (1 << PS::R) + j] = lds[swz(PS::index(t, g, j))];
+}
+
+// Twiddles of one pass, loaded up front: stage with local bit LB uses the
+// 2^LB values at flat positions 2^LB - 1 + jl (jl < 2^LB), as (value, Shoup)
+// pairs.  They do not depend on the group (BLO <= log2 T), only on t.
+template <typename W, int L, int LOGE, int S0>
+__device__ __forceinline__ void load_pass_tw(W (&tw)[2 * ((1 << LOGE) - 1)], int t,
+                                             const W *__restrict__ stab)
+{
+    using PS = PassShape<L, LOGE, S0>;
+    constexpr int R = PS::R;
+    const int glo = t & ((1 << PS::BLO) - 1);
+#pragma unroll
+    for (int f = 0; f < (1 << R) - 1; ++f) {
+        const int LB = 31 - __builtin_clz(f + 1);
+        const int jl = f + 1 - (1 << LB);
+        const int s = S0 + (R - 1 - LB);
+        const int off = (1 << L) - (1 << (L - s));
+        load_tw<W>(stab, off + ((jl << PS::BLO) | glo), tw[2 * f], tw[2 * f + 1]);
+    }
+}
+
+// forward (DIF) stages of the pass at S0, local bit LB down to 0
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int LB>
+__device__ __forceinline__ void fwd_stages(W (&x)[1 << LOGE], const W (&tw)[2 * ((1 << LOGE) - 1)],
+                                           const Bfly<W> &k)
+{
+    using PS = PassShape<L, LOGE, S0>;
+    constexpr int R = PS::R;
+#pragma unroll
+    for (int g = 0; g < PS::G; ++g) {
+#pragma unroll
+        for (int jl = 0; jl < (1 << LB); ++jl) {
+            const W w = tw[2 * ((1 << LB) - 1 + jl)], wq = tw[2 * ((1 << LB) - 1 + jl) + 1];
+#pragma unroll
+            for (int jh = 0; jh < (1 << (R - 1 - LB)); ++jh) {
+                const int j = (jh << (LB + 1)) | jl;
+                bfly_dif<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
+            }
+        }
+    }
+    if constexpr (LB > 0)
+        fwd_stages<W, LOOSE, L, LOGE, S0, LB - 1>(x, tw, k);
+}
+
+// inverse (DIT) stages of the pass at S0, local bit LB up to R-1
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int LB>
+__device__ __forceinline__ void inv_stages(W (&x)[1 << LOGE], const W (&tw)[2 * ((1 << LOGE) - 1)],
+                                           const Bfly<W> &k)
+{
+    using PS = PassShape<L, LOGE, S0>;
+    constexpr int R = PS::R;
+#pragma unroll
+    for (int g = 0; g < PS::G; ++g) {
+#pragma unroll
+        for (int jl = 0; jl < (1 << LB); ++jl) {
+            const W w = tw[2 * ((1 << LB) - 1 + jl)], wq = tw[2 * ((1 << LB) - 1 + jl) + 1];
+#pragma unroll
+            for (int jh = 0; jh < (1 << (R - 1 - LB)); ++jh) {
+                const int j = (jh << (LB + 1)) | jl;
+                bfly_dit<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
+            }
+        }
+    }
+    if constexpr (LB + 1 < R)
+        inv_stages<W, LOOSE, L, LOGE, S0, LB + 1>(x, tw, k);
+}
+
+// forward passes from S0 to the end (x holds the pass-S0 arrangement and tw
+// its twiddles); on return the data sits in LDS in natural (DIF output) order
+template <typename W, bool LOOSE, int L, int LOGE, int S0>
+__device__ __forceinline__ void fwd_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 << LOGE) - 1)],
+                                           W *lds, int t, const W *__restrict__ stw,
+                                           const Bfly<W> &k)
+{
+    using PS = PassShape<L, LOGE, S0>;
+    fwd_stages<W, LOOSE, L, LOGE, S0, PS::R - 1>(x, tw, k);
+    if constexpr (S0 + PS::R < L)
+        load_pass_tw<W, L, LOGE, S0 + PS::R>(tw, t, stw);  // in flight during the exchange
+    lds_store<W, L, LOGE, S0>(x, lds, t);
+    __syncthreads();
+    if constexpr (S0 + PS::R < L) {
+        lds_load<W, L, LOGE, S0 + PS::R>(x, lds, t);
+        __syncthreads();
+        fwd_passes<W, LOOSE, L, LOGE, S0 + PS::R>(x, tw, lds, t, stw, k);
+    }
+}
+
+// inverse passes from S0 down to 0; on return x holds the pass-0 arrangement
+template <typename W, bool LOOSE, int L, int LOGE, int S0>
+__device__ __forceinline__ void inv_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 << LOGE) - 1)],
+                                           W *lds, int t, const W *__restrict__ sitw,
+                                           const Bfly<W> &k)
+{
+    inv_stages<W, LOOSE, L, LOGE, S0, 0>(x, tw, k);
+    if constexpr (S0 > 0) {
+        load_pass_tw<W, L, LOGE, S0 - LOGE>(tw, t, sitw);
+        __syncthreads();  // every thread has finished reading the previous arrangement
+        lds_store<W, L, LOGE, S0>(x, lds, t);
+        __syncthreads();
+        lds_load<W, L, LOGE, S0 - LOGE>(x, lds, t);
+        inv_passes<W, LOOSE, L, LOGE, S0 - LOGE>(x, tw, lds, t, sitw, k);
+    }
+}
+
+template <typename Win, typename W, bool LOOSE, int L, int LOGE>
+__global__ void __launch_bounds__(1 << (L - LOGE))
+ntt_fwd_reg(const Win *__restrict__ in, int64_t in_stride, int64_t in_len, W *__restrict__ out,
+            int64_t units, int logR, int logP, NttTab<W> tab, W p)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    W *lds = reinterpret_cast<W *>(smem_raw);
+    constexpr int E = 1 << LOGE, T = 1 << (L - LOGE), M = 1 << L;
+    const int t = threadIdx.x;
+    const int64_t u = xcd_unit(blockIdx.x, units);
+    const int64_t e = u >> logR;
+    const int R = 1 << logR;
+    const int gg = (int)(u & (R - 1));
+    const int logN = L + logR;
+    const int64_t N = (int64_t)1 << logN;
+    const W p2 = p + p;
+    const Win *xin = in + e * in_stride;
+
+    W tw[2 * (E - 1)];
+    load_pass_tw<W, L, LOGE, 0>(tw, t, tab.stw);
+    W x[E];
+    // pass-0 arrangement: element j of thread t is index (j << (L-LOGE)) | t
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const int i1 = (j << (L - LOGE)) | t;
+        W v = 0;
+        if (R == 1) {
+            if (i1 < in_len)
+                v = (W)xin[i1];
+        } else {
+            W acc = 0;
+            for (int t2 = 0; t2 < R; ++t2) {
+                const int64_t src = (int64_t)i1 + (int64_t)M * t2;
+                if (src < in_len) {
+                    const int64_t widx = ((int64_t)M * gg * t2) & (N - 1);
+                    acc = add_2p<W>(acc, shoup_lazy((W)xin[src], tab.tw[widx], tab.twq[widx], p), p2);
+                }
+            }
+            const int64_t widx = (int64_t)i1 * gg;
+            v = shoup_lazy(acc, tab.tw[widx], tab.twq[widx], p);
+        }
+        x[j] = v;
+    }
+    const Bfly<W> k(p);
+    fwd_passes<W, LOOSE, L, LOGE, 0>(x, tw, lds, t, tab.stw, k);
+    const int64_t count = units >> logR;
+#pragma unroll
+    for (int v = 0; v < E; ++v) {
+        const int idx = t + T * v;
+        out[blk_addr(e, (int64_t)gg * M + idx, count, logP)] = bfly_final<W, LOOSE>(lds[swz(idx)], k);
+    }
+}
+
+template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
+__global__ void __launch_bounds__(1 << (L - LOGE))
+ntt_inv_reg(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
+            int64_t hi, W *__restrict__ tmp, int64_t units, int logR, int logP, NttTab<W> tab, W p)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    W *lds = reinterpret_cast<W *>(smem_raw);
+    constexpr int E = 1 << LOGE, T = 1 << (L - LOGE), M = 1 << L;
+    const int t = threadIdx.x;
+    const int64_t u = xcd_unit(blockIdx.x, units);
+    const int64_t e = u >> logR;
+    const int R = 1 << logR;
+    const int gg = (int)(u & (R - 1));
+    const int64_t N = (int64_t)M << logR;
+    const int64_t count = units >> logR;
+    constexpr int SLAST = ((L - 1) / LOGE) * LOGE;
+    W tw[2 * (E - 1)];
+    load_pass_tw<W, L, LOGE, SLAST>(tw, t, tab.sitw);
+#pragma unroll
+    for (int v = 0; v < E; ++v) {
+        const int idx = t + T * v;
+        lds[swz(idx)] = in[blk_addr(e, (int64_t)gg * M + idx, count, logP)];
+    }
+    __syncthreads();
+    W x[E];
+    lds_load<W, L, LOGE, SLAST>(x, lds, t);
+    const Bfly<W> k(p);
+    inv_passes<W, LOOSE, L, LOGE, SLAST>(x, tw, lds, t, tab.sitw, k);
+    // pass-0 arrangement: element j is index (j << (L-LOGE)) | t
+    if (R == 1) {
+        Wout *o = out + e * out_stride;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int i1 = (j << (L - LOGE)) | t;
+            if (i1 >= lo && i1 < hi)
+                o[i1 - lo] = (Wout)shoup<W>(x[j], tab.ninv, tab.ninvq, p);
+        }
+    } else {
+        W *o = tmp + e * N + (int64_t)gg * M;
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            o[(j << (L - LOGE)) | t] = bfly_final<W, LOOSE>(x[j], k);
+    }
+}
+
+// Register-kernel launchers.  logE (elements per thread) is a tuning
+// parameter: 4 (radix-16 passes, more threads, fewer registers) or 5
+// (radix-32 passes, fewer LDS exchanges).
+template <typename Win, typename W, bool LOOSE, int L, int LOGE>
+hipError_t launch_fwd_reg3(const Win *in, int64_t in_stride, int64_t in_len, W *out,
+                           int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
+                           hipStream_t s)
+{
+    constexpr int T = 1 << (L - LOGE);
+    const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)ntt_fwd_reg<Win, W, LOOSE, L, LOGE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((ntt_fwd_reg<Win, W, LOOSE, L, LOGE>), dim3((unsigned)units), dim3(T), lds,
+                       s, in, in_stride, in_len, out, units, logR, logP, tab, p);
+    return hipGetLastError();
+}
+
+template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
+hipError_t launch_inv_reg3(const W *in, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
+                           W *tmp, int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
+                           hipStream_t s)
+{
+    constexpr int T = 1 << (L - LOGE);
+    const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)ntt_inv_reg<W, Wout, LOOSE, L, LOGE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((ntt_inv_reg<W, Wout, LOOSE, L, LOGE>), dim3((unsigned)units), dim3(T), lds,
+                       s, in, out, out_stride, lo, hi, tmp, units, logR, logP, tab, p);
+    return hipGetLastError();
+}
+
+template <typename Win, typename W, bool LOOSE, int L>
+hipError_t launch_fwd_reg2(int logE, const Win *in, int64_t in_stride, int64_t in_len, W *out,
+                           int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
+                           hipStream_t s)
+{
+    if (logE == 5 && L >= 12)
+        return launch_fwd_reg3<Win, W, LOOSE, L, (L >= 12 ? 5 : 4)>(in, in_stride, in_len, out,
+                                                                   units, logR, logP, tab, p, s);
+    return launch_fwd_reg3<Win, W, LOOSE, L, 4>(in, in_stride, in_len, out, units, logR, logP, tab,
+                                                p, s);
+}
+
+template <typename W, typename Wout, bool LOOSE, int L>
+hipError_t launch_inv_reg2(int logE, const W *in, Wout *out, int64_t out_stride, int64_t lo,
+                           int64_t hi, W *tmp, int64_t units, int logR, int logP,
+                           const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if (logE == 5 && L >= 12)
+        return launch_inv_reg3<W, Wout, LOOSE, L, (L >= 12 ? 5 : 4)>(in, out, out_stride, lo, hi,
+                                                                    tmp, units, logR, logP, tab,
+                                                                    p, s);
+    return launch_inv_reg3<W, Wout, LOOSE, L, 4>(in, out, out_stride, lo, hi, tmp, units, logR,
+                                                 logP, tab, p, s);
+}
+
+template <typename Win, typename W, int L>
+hipError_t launch_fwd_reg(int logE, const Win *in, int64_t in_stride, int64_t in_len, W *out,
+                          int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
+                          hipStream_t s)
+{
+    if constexpr (sizeof(W) == 8) {
+        if ((uint64_t)p < ((uint64_t)1 << 61))
+            return launch_fwd_reg2<Win, W, true, L>(logE, in, in_stride, in_len, out, units, logR,
+                                                    logP, tab, p, s);
+    }
+    return launch_fwd_reg2<Win, W, false, L>(logE, in, in_stride, in_len, out, units, logR, logP,
+                                             tab, p, s);
+}
+
+template <typename W, typename Wout, int L>
+hipError_t launch_inv_reg(int logE, const W *in, Wout *out, int64_t out_stride, int64_t lo,
+                          int64_t hi, W *tmp, int64_t units, int logR, int logP,
+                          const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if constexpr (sizeof(W) == 8) {
+        if ((uint64_t)p < ((uint64_t)1 << 61))
+            return launch_inv_reg2<W, Wout, true, L>(logE, in, out, out_stride, lo, hi, tmp, units,
+                                                     logR, logP, tab, p, s);
+    }
+    return launch_inv_reg2<W, Wout, false, L>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
+                                              logP, tab, p, s);
+}
+
+template <typename Win, typename W>
+hipError_t dispatch_fwd_reg(int logM, int logE, const Win *in, int64_t in_stride, int64_t in_len,
+                            W *out, int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
+                            hipStream_t s)
+{
+    switch (logM) {
+    case 11:
+        return launch_fwd_reg<Win, W, 11>(logE, in, in_stride, in_len, out, units, logR, logP, tab, p, s);
+    case 12:
+        return launch_fwd_reg<Win, W, 12>(logE, in, in_stride, in_len, out, units, logR, logP, tab, p, s);
+    case 13:
+        return launch_fwd_reg<Win, W, 13>(logE, in, in_stride, in_len, out, units, logR, logP, tab, p, s);
+    case 14:
+        if constexpr (sizeof(W) == 4)
+            return launch_fwd_reg<Win, W, 14>(logE, in, in_stride, in_len, out, units, logR, logP,
+                                              tab, p, s);
+        [[fallthrough]];
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+template <typename W, typename Wout>
+hipError_t dispatch_inv_reg(int logM, int logE, const W *in, Wout *out, int64_t out_stride,
+                            int64_t lo, int64_t hi, W *tmp, int64_t units, int logR, int logP,
+                            const NttTab<W> &tab, W p, hipStream_t s)
+{
+    switch (logM) {
+    case 11:
+        return launch_inv_reg<W, Wout, 11>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
+                                           logP, tab, p, s);
+    case 12:
+        return launch_inv_reg<W, Wout, 12>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
+                                           logP, tab, p, s);
+    case 13:
+        return launch_inv_reg<W, Wout, 13>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
+                                           logP, tab, p, s);
+    case 14:
+        if constexpr (sizeof(W) == 4)
+            return launch_inv_reg<W, Wout, 14>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
+                                               logP, tab, p, s);
+        [[fallthrough]];
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+constexpr int kRegMinLog = 11;
+
 }  // namespace
 
 template <typename Win, typename W>
 hipError_t launch_ntt_fwd(const Win *in, int64_t in_stride, int64_t in_len, W *out,
-                          int64_t count, int logN, const NttTab<W> &tab, W p, hipStream_t s)
+                          int64_t count, int logN, int logP, const NttTune &tune,
+                          const NttTab<W> &tab, W p, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     const int logM = logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
     const int logR = logN - logM;
-    if (logR > 2)
+    if (logR > 2 || logP > logN)
         return hipErrorInvalidValue;
-    const int lognb = pick_lognb(logM);
     const int64_t units = count << logR;
+    if (logM >= kRegMinLog)
+        return dispatch_fwd_reg<Win, W>(logM, tune.fwd_log_e, in, in_stride, in_len, out, units,
+                                        logR, logP, tab, p, s);
+    const int lognb = pick_lognb(logM);
     const int64_t blocks = (units + (1 << lognb) - 1) >> lognb;
     const size_t lds = ((size_t)1 << (logM + lognb)) * sizeof(W);
     hipLaunchKernelGGL((ntt_fwd_kernel<Win, W>), dim3((unsigned)blocks), dim3(kThreads), lds, s, in,
-                       in_stride, in_len, out, units, logN, logR, lognb, tab, p);
+                       in_stride, in_len, out, units, logN, logR, lognb, logP, tab, p);
     return hipGetLastError();
 }
 
 template <typename W, typename Wout>
 hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
-                          int64_t count, int logN, const NttTab<W> &tab, W p, W *tmp,
-                          hipStream_t s)
+                          int64_t count, int logN, int logP, const NttTune &tune,
+                          const NttTab<W> &tab, W p, W *tmp, hipStream_t s)
 {
     const int64_t N = (int64_t)1 << logN;
     if (hi > N)
@@ -233,15 +741,22 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
         return hipSuccess;
     const int logM = logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
     const int logR = logN - logM;
-    if (logR > 2 || (logR > 0 && tmp == nullptr))
+    if (logR > 2 || (logR > 0 && tmp == nullptr) || logP > logN)
         return hipErrorInvalidValue;
-    const int lognb = pick_lognb(logM);
     const int64_t units = count << logR;
-    const int64_t blocks = (units + (1 << lognb) - 1) >> lognb;
-    const size_t lds = ((size_t)1 << (logM + lognb)) * sizeof(W);
-    hipLaunchKernelGGL((ntt_inv_kernel<W, Wout>), dim3((unsigned)blocks), dim3(kThreads), lds, s,
-                       in, out, out_stride, lo, hi, tmp, units, logN, logR, lognb, tab, p);
-    hipError_t err = hipGetLastError();
+    hipError_t err;
+    if (logM >= kRegMinLog) {
+        err = dispatch_inv_reg<W, Wout>(logM, tune.inv_log_e, in, out, out_stride, lo, hi, tmp,
+                                        units, logR, logP, tab, p, s);
+    } else {
+        const int lognb = pick_lognb(logM);
+        const int64_t blocks = (units + (1 << lognb) - 1) >> lognb;
+        const size_t lds = ((size_t)1 << (logM + lognb)) * sizeof(W);
+        hipLaunchKernelGGL((ntt_inv_kernel<W, Wout>), dim3((unsigned)blocks), dim3(kThreads), lds,
+                           s, in, out, out_stride, lo, hi, tmp, units, logN, logR, lognb, logP,
+                           tab, p);
+        err = hipGetLastError();
+    }
     if (err != hipSuccess || logR == 0)
         return err;
     const int64_t total = count * (hi - lo);
@@ -254,20 +769,20 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
 }
 
 template hipError_t launch_ntt_fwd<uint64_t, uint64_t>(const uint64_t *, int64_t, int64_t,
-                                                       uint64_t *, int64_t, int,
-                                                       const NttTab<uint64_t> &, uint64_t,
-                                                       hipStream_t);
+                                                       uint64_t *, int64_t, int, int,
+                                                       const NttTune &, const NttTab<uint64_t> &,
+                                                       uint64_t, hipStream_t);
 template hipError_t launch_ntt_fwd<uint64_t, uint32_t>(const uint64_t *, int64_t, int64_t,
-                                                       uint32_t *, int64_t, int,
-                                                       const NttTab<uint32_t> &, uint32_t,
-                                                       hipStream_t);
+                                                       uint32_t *, int64_t, int, int,
+                                                       const NttTune &, const NttTab<uint32_t> &,
+                                                       uint32_t, hipStream_t);
 template hipError_t launch_ntt_inv<uint64_t, uint64_t>(const uint64_t *, uint64_t *, int64_t,
-                                                       int64_t, int64_t, int64_t, int,
-                                                       const NttTab<uint64_t> &, uint64_t,
-                                                       uint64_t *, hipStream_t);
+                                                       int64_t, int64_t, int64_t, int, int,
+                                                       const NttTune &, const NttTab<uint64_t> &,
+                                                       uint64_t, uint64_t *, hipStream_t);
 template hipError_t launch_ntt_inv<uint32_t, uint64_t>(const uint32_t *, uint64_t *, int64_t,
-                                                       int64_t, int64_t, int64_t, int,
-                                                       const NttTab<uint32_t> &, uint32_t,
-                                                       uint32_t *, hipStream_t);
+                                                       int64_t, int64_t, int64_t, int, int,
+                                                       const NttTune &, const NttTab<uint32_t> &,
+                                                       uint32_t, uint32_t *, hipStream_t);
 
 }  // namespace pmlg

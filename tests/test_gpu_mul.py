@@ -151,3 +151,22 @@ def test_config2_shape_property(pml):
     # one entry in full against the closed form
     c00 = O.polmat_mul(A[:1], B[:, :1], p, method="ntt")
     assert np.array_equal(C1[:1, :1], c00)
+
+
+def test_mfma_inner_dimension_bound(pml):
+    """The int8-limb kernel rebuilds the exact dot product V in 128 bits and
+    reduces it by Montgomery REDC, which needs V + (p+1) 2^64 <= 2^128, i.e.
+    k <= 239 for this prime; beyond, 'auto' uses the plain kernel and 'mfma'
+    refuses."""
+    rng = np.random.default_rng(17)
+    p = P60
+    ctx = ctx_for(pml, p)
+    A, B = rand(rng, (3, 239, 9), p), rand(rng, (239, 2, 7), p)
+    assert np.array_equal(ctx.mul(A, B, method="mfma"), O.polmat_mul(A, B, p))
+    # worst case for the accumulators: every coefficient p-1
+    A, B = np.full((2, 239, 3), p - 1, np.uint64), np.full((239, 2, 3), p - 1, np.uint64)
+    assert np.array_equal(ctx.mul(A, B, method="mfma"), O.polmat_mul(A, B, p))
+    A, B = rand(rng, (2, 300, 5), p), rand(rng, (300, 2, 5), p)
+    assert np.array_equal(ctx.mul(A, B, method="auto"), O.polmat_mul(A, B, p))
+    with pytest.raises(pml.PmlgError):
+        ctx.mul(A, B, method="mfma")
