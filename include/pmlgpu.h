@@ -88,6 +88,26 @@ pmlg_status pmlg_polmat_mulmid_dev(pmlg_ctx ctx, uint64_t *dout, const uint64_t 
                                    int64_t lenA, int64_t lenB, int64_t nlo, int64_t nhi,
                                    int method, void *stream);
 
+/* Shift-ordered weak Popov approximant basis of F (m x n, padded length lenF)
+ * at order `order` (same order for every column), by divide and conquer on the
+ * order (nmod_poly_mat_pmbasis) or iteratively (nmod_poly_mat_mbasis).
+ * P receives m x m entries of capacity capP >= order + 1 (entry (i,j)
+ * coefficient d at P[(i*m + j)*capP + d]); *lenP its normalised length;
+ * shift (length m) is the input shift on entry and the shifted row degree of
+ * P on return.  1 <= m, n <= 128.  The products inside PM-Basis run on the
+ * device with the given pointwise method. */
+pmlg_status pmlg_pmbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
+                         const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
+                         int64_t capP, int method);
+pmlg_status pmlg_mbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
+                        const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
+                        int64_t capP);
+/* PM-Basis with F and P in device memory (P capacity capP >= order + 1, its
+ * length in *lenP is the computed one, not normalised); shift stays on the host. */
+pmlg_status pmlg_pmbasis_dev(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP,
+                             int64_t *shift, const uint64_t *dF, int64_t m, int64_t n,
+                             int64_t lenF, int64_t order, int method, void *stream);
+
 /* Per-stage device timing (HIP events around each launch).  Stages:
  * 0 forward NTT, 1 pointwise product, 2 inverse NTT (+ merge). */
 void pmlg_profile_enable(pmlg_ctx ctx, int on);

@@ -6,13 +6,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pmlgpu.h"
-#include "internal.hpp"
+#include "ctx.hpp"
 #include "host_arith.hpp"
 
 using namespace pmlg;
 
-namespace {
+namespace pmlg {
 
 thread_local std::string g_last_error;
 
@@ -27,93 +26,14 @@ pmlg_status hip_fail(hipError_t e, const char *where)
     return fail(PMLG_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr, where)                \
-    do {                                    \
-        hipError_t _e = (expr);             \
-        if (_e != hipSuccess)               \
-            return hip_fail(_e, where);     \
-    } while (0)
 
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t b)
-    {
-        if (b <= bytes)
-            return hipSuccess;
-        if (ptr) {
-            // work enqueued earlier (possibly on a caller's stream) may still read it
-            (void)hipDeviceSynchronize();
-            (void)hipFree(ptr);
-        }
-        ptr = nullptr;
-        bytes = 0;
-        size_t want = b + b / 4;  // some headroom against regrowth
-        hipError_t e = hipMalloc(&ptr, want);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            return e;
-        }
-        bytes = want;
-        return hipSuccess;
-    }
-    void release()
-    {
-        if (ptr)
-            (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-};
 
-struct Tables {
-    void *mem = nullptr;
-    NttTab<uint64_t> t64{};
-    NttTab<uint32_t> t32{};
-};
 
-struct Prof {
-    int stage;
-    hipEvent_t a, b;
-};
 
-}  // namespace
+}  // namespace pmlg
 
-struct pmlg_ctx_s {
-    uint64_t p = 0;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    ModConst mc{};
-    bool small = false;  // p < 2^30: 32-bit words on the device
-    int max_log = 0;     // log2 of the largest supported transform
-    uint64_t root = 0;   // primitive 2^max_log-th root of unity
-    std::vector<Tables> tabs;
-    DevBuf in_a, in_b, out_c, ev_a, ev_b, ev_c, tmp;
-    std::mutex mu;
-    NttTune tune;
-    bool profile = false;
-    std::vector<Prof> pending;
-    double ms[3] = {0, 0, 0};
-    int64_t launches[3] = {0, 0, 0};
-};
+namespace pmlg {
 
-namespace {
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev) != hipSuccess)
-            prev = -1;
-        if (prev != dev)
-            (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard()
-    {
-        if (prev >= 0)
-            (void)hipSetDevice(prev);
-    }
-};
 
 int ceil_log2(int64_t x)
 {
@@ -370,10 +290,12 @@ pmlg_status pick_kind(pmlg_ctx ctx, int method, int64_t m, int64_t k, int64_t n,
 }
 
 // Core of every product call: coefficients [nlo, nhi) of A*B into dOut
-// (padded length nhi - nlo), all pointers on the device.
-pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
-                              const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t lenA,
-                              int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s)
+// (entry e at dOut + e*out_stride, out_stride >= nhi - nlo), all pointers on
+// the device; entry e of A at dA + e*strideA with lenA <= strideA (same for B).
+pmlg_status mulmid_dev_strided(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride,
+                               const uint64_t *dA, int64_t strideA, const uint64_t *dB,
+                               int64_t strideB, int64_t m, int64_t k, int64_t n, int64_t lenA,
+                               int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s)
 {
     if (m < 0 || k < 0 || n < 0 || lenA < 0 || lenB < 0 || nlo < 0)
         return fail(PMLG_ERR_ARG, "negative dimension, length or nlo");
@@ -382,12 +304,14 @@ pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
     const int64_t width = nhi - nlo;
     if (width <= 0 || m == 0 || n == 0)
         return PMLG_OK;
+    if (out_stride < width || strideA < lenA || strideB < lenB)
+        return fail(PMLG_ERR_ARG, "stride shorter than the length it holds");
     if ((!dOut) || ((k > 0 && lenA > 0 && lenB > 0) && (!dA || !dB)))
         return fail(PMLG_ERR_ARG, "null buffer");
     const int64_t lenC = (lenA > 0 && lenB > 0) ? lenA + lenB - 1 : 0;
     const int64_t hi_eff = nhi < lenC ? nhi : lenC;
     if (k == 0 || hi_eff <= nlo) {
-        HIP_TRY(fill_zero(dOut, m * n * width, s), "fill_zero");
+        HIP_TRY(fill_zero(dOut, m * n * out_stride, s), "fill_zero");
         return PMLG_OK;
     }
     // Only coefficients below hi_eff of each operand matter; they are read in
@@ -407,13 +331,21 @@ pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
     if (st != PMLG_OK)
         return st;
     if (hi_eff < nhi)  // coefficients beyond the product length are zero
-        HIP_TRY(fill_zero(dOut, m * n * width, s), "fill_zero");
-    // operands keep their strides lenA / lenB; la/lb only bound what is read
+        HIP_TRY(fill_zero(dOut, m * n * out_stride, s), "fill_zero");
     if (ctx->small)
-        return run_product<uint32_t>(ctx, dOut, width, dA, dB, m, k, n, lenA, lenB, la, lb, nlo,
-                                     hi_eff, logN, kind, s);
-    return run_product<uint64_t>(ctx, dOut, width, dA, dB, m, k, n, lenA, lenB, la, lb, nlo,
-                                 hi_eff, logN, kind, s);
+        return run_product<uint32_t>(ctx, dOut, out_stride, dA, dB, m, k, n, strideA, strideB, la,
+                                     lb, nlo, hi_eff, logN, kind, s);
+    return run_product<uint64_t>(ctx, dOut, out_stride, dA, dB, m, k, n, strideA, strideB, la, lb,
+                                 nlo, hi_eff, logN, kind, s);
+}
+
+pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
+                              const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t lenA,
+                              int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s)
+{
+    const int64_t width = nhi - nlo;
+    return mulmid_dev_strided(ctx, dOut, width > 0 ? width : 0, dA, lenA, dB, lenB, m, k, n, lenA,
+                              lenB, nlo, nhi, method, s);
 }
 
 }  // namespace
@@ -488,6 +420,7 @@ void pmlg_ctx_destroy(pmlg_ctx ctx)
         ctx->ev_b.release();
         ctx->ev_c.release();
         ctx->tmp.release();
+        ctx->arena.release();
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -1,0 +1,118 @@
+// Context object shared by the host translation units of libpmlgpu.
+#pragma once
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pmlgpu.h"
+#include "internal.hpp"
+
+namespace pmlg {
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes)
+            return hipSuccess;
+        if (ptr) {
+            // work enqueued earlier (possibly on a caller's stream) may still read it
+            (void)hipDeviceSynchronize();
+            (void)hipFree(ptr);
+        }
+        ptr = nullptr;
+        bytes = 0;
+        size_t want = b + b / 4;  // some headroom against regrowth
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        bytes = want;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+struct Tables {
+    void *mem = nullptr;
+    NttTab<uint64_t> t64{};
+    NttTab<uint32_t> t32{};
+};
+
+struct Prof {
+    int stage;
+    hipEvent_t a, b;
+};
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        if (prev != dev)
+            (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+    }
+};
+
+pmlg_status fail(pmlg_status st, const std::string &msg);
+pmlg_status hip_fail(hipError_t e, const char *where);
+
+#define HIP_TRY(expr, where)                \
+    do {                                    \
+        hipError_t _e = (expr);             \
+        if (_e != hipSuccess)               \
+            return hip_fail(_e, where);     \
+    } while (0)
+
+}  // namespace pmlg
+
+struct pmlg_ctx_s {
+    uint64_t p = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pmlg::ModConst mc{};
+    bool small = false;  // p < 2^30: 32-bit words on the device
+    int max_log = 0;     // log2 of the largest supported transform
+    uint64_t root = 0;   // primitive 2^max_log-th root of unity
+    std::vector<pmlg::Tables> tabs;
+    pmlg::DevBuf in_a, in_b, out_c, ev_a, ev_b, ev_c, tmp, arena;
+    std::mutex mu;
+    pmlg::NttTune tune;
+    bool profile = false;
+    std::vector<pmlg::Prof> pending;
+    double ms[3] = {0, 0, 0};
+    int64_t launches[3] = {0, 0, 0};
+};
+
+namespace pmlg {
+
+int ceil_log2(int64_t x);
+pmlg_status check_ctx(pmlg_ctx ctx);
+hipError_t fill_zero(uint64_t *p, int64_t n, hipStream_t s);
+// Coefficients [nlo, nhi) of A*B into dOut (padded length nhi - nlo), device
+// pointers, operand strides = their padded lengths; ctx->mu must be held.
+pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
+                              const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t lenA,
+                              int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s);
+// Same with explicit operand strides (entry e of A at dA + e*strideA, length lenA <= strideA).
+pmlg_status mulmid_dev_strided(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride,
+                               const uint64_t *dA, int64_t strideA, const uint64_t *dB,
+                               int64_t strideB, int64_t m, int64_t k, int64_t n, int64_t lenA,
+                               int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s);
+
+}  // namespace pmlg

@@ -1,0 +1,656 @@
+// Approximant bases on the device: M-Basis (iterative, one order per step)
+// and PM-Basis (divide and conquer on the order, with the device middle
+// product and product), following
+//   flint-extras/src/nmod_mat_poly_extra/nmod_mat_poly_mbasis.c:116-378
+//   flint-extras/src/nmod_poly_mat_extra/approximant_basis.c:18-69
+//   flint-extras/src/nmod_mat_extra/left_nullspace.c:42-102
+//
+// One M-Basis step at order `ord`:
+//  1. residual = coefficient ord of appbas * F.  It is kept as a truncated
+//     series R = appbas * F mod x^order, updated by the same row operations
+//     as appbas (the value is unique, so this equals the reference's
+//     nmod_mat_poly_mul_coeff);
+//  2. perm = stable sort of (shift[i], i); residual rows permuted by it;
+//  3. left nullspace of the permuted residual in compact form: from the
+//     reduced row echelon form R of its transpose (computed fraction-free,
+//     then each pivot row divided by its pivot), X[t][r] = -R[r][nonpivot_t],
+//     permutation = (pivot columns) ++ (non-pivot columns);
+//  4. nullity 0: appbas <- x^(order-ord) appbas, shift += order-ord, stop;
+//     nullity m: nothing; otherwise with piv[i] = perm[permutation[i]]:
+//     shift[piv[r]] += 1 (r < rank), row piv[rank+t] += sum_r X[t][r] row piv[r],
+//     rows piv[r] multiplied by x (length grows when one of them reaches the top).
+//
+// Kernel 1 (one workgroup) does 2-4 and writes the step's description;
+// kernel 2 (a grid) applies the row operations to appbas and R, reading one
+// buffer and writing the other.  All control (nullity cases, basis length)
+// stays on the device: a leaf is a fixed sequence of launches.
+#include <algorithm>
+#include <cstring>
+
+#include "ctx.hpp"
+#include "host_arith.hpp"
+
+namespace pmlg {
+
+namespace {
+
+constexpr int kStepThreads = 1024;
+constexpr int kMaxDim = 128;  // m, n limits of the device M-Basis step
+
+// control words of one leaf
+enum Ctl {
+    C_CUR = 0,    // index (0/1) of the buffers holding the current appbas and R
+    C_MODE = 1,   // what kernel 2 does at this step: 0 nothing, 1 update, 2 shift all
+    C_RANK = 2,   // rank of the residual at this step
+    C_LEN = 3,    // current length of appbas
+    C_GROW = 4,   // set by kernel 2 when the length grows by one
+    C_DONE = 5,   // full-rank residual met: the leaf is finished
+    C_SHIFT = 6,  // mode 2: power of x
+    C_LOLD = 7,   // mode 2: length before the shift
+    C_FLIP = 8,   // kernel 2 wrote the other buffers: swap before the next step
+    C_WORDS = 16
+};
+
+struct LeafArgs {
+    int m, n;
+    int ordL;     // order of the leaf
+    int cap;      // coefficient capacity of appbas (ordL + 1)
+    uint64_t *app[2];  // m x m x cap, entry-major
+    uint64_t *R[2];    // m x n x ordL, entry-major
+    uint64_t *X;       // nullity x rank, row stride m
+    int *piv;          // m
+    int *rowinfo;      // m: t >= 0 nullity row t, -1-r rank row r
+    int64_t *shift;    // m
+    int *ctl;          // C_WORDS
+    ModConst mc;
+};
+
+__device__ __forceinline__ uint64_t mulmod_g(uint64_t a, uint64_t b, const ModConst &mc)
+{
+    if (mc.p < ((uint64_t)1 << 32))
+        return barrett(a * b, mc.bm, mc.p);
+    return reduce128(mulhi64(a, b), a * b, mc);
+}
+
+__device__ __forceinline__ uint64_t submod_g(uint64_t a, uint64_t b, uint64_t p)
+{
+    return a >= b ? a - b : a + (p - b);
+}
+
+__device__ uint64_t invmod_g(uint64_t a, const ModConst &mc)
+{
+    // Fermat: a^(p-2)
+    uint64_t e = mc.p - 2, r = 1, b = a;
+    while (e) {
+        if (e & 1)
+            r = mulmod_g(r, b, mc);
+        b = mulmod_g(b, b, mc);
+        e >>= 1;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// kernel 1: permutation, left nullspace, step description
+// ---------------------------------------------------------------------------
+// Fraction-free Gauss-Jordan on At = (permuted residual)^T (n x m): pivot rows
+// stay where they are (prow[k] is the row of the k-th pivot, pivcol[k] its
+// column); for a pivot at (pr, c) every other row r becomes
+// pv * row_r - At[r][c] * row_pr with pv = At[pr][c].  The k-th row of the
+// reduced row echelon form is row prow[k] divided by its pivot.
+__global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, int ord)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int m = a.m, n = a.n;
+    uint64_t *At = reinterpret_cast<uint64_t *>(smem);           // n x m
+    uint64_t *fcol = At + (size_t)n * m;                         // kMaxDim
+    uint64_t *inv = fcol + kMaxDim;                              // kMaxDim
+    int *perm = reinterpret_cast<int *>(inv + kMaxDim);          // kMaxDim
+    int *pivcol = perm + kMaxDim;                                // kMaxDim
+    int *plist = pivcol + kMaxDim;                               // kMaxDim
+    int *prow = plist + kMaxDim;                                 // kMaxDim
+    int *ispiv = prow + kMaxDim;                                 // kMaxDim
+    int64_t *sh = reinterpret_cast<int64_t *>(ispiv + kMaxDim);  // kMaxDim
+    __shared__ int s_state[4];
+    const int tid = threadIdx.x;
+    const uint64_t p = a.mc.p;
+
+    if (tid == 0) {
+        int *c = a.ctl;
+        if (c[C_FLIP]) {
+            c[C_CUR] ^= 1;
+            c[C_FLIP] = 0;
+        }
+        c[C_LEN] += c[C_GROW];
+        c[C_GROW] = 0;
+        if (c[C_DONE])
+            c[C_MODE] = 0;
+        s_state[0] = c[C_DONE];
+        s_state[1] = c[C_CUR];
+    }
+    for (int i = tid; i < m; i += kStepThreads)
+        sh[i] = a.shift[i];
+    for (int r = tid; r < n; r += kStepThreads)
+        ispiv[r] = 0;
+    __syncthreads();
+    if (s_state[0])
+        return;
+    const int cur = s_state[1];
+    const uint64_t *R = a.R[cur];
+
+    // stable permutation sorting (shift, index)
+    for (int i = tid; i < m; i += kStepThreads) {
+        const int64_t si = sh[i];
+        int pos = 0;
+        for (int j = 0; j < m; ++j) {
+            const int64_t sj = sh[j];
+            pos += (sj < si) || (sj == si && j < i);
+        }
+        perm[pos] = i;
+    }
+    __syncthreads();
+    // this thread's elements of At: column col, rows r0, r0 + rstep, ...
+    const int rstep = kStepThreads / m;  // m <= kMaxDim <= kStepThreads
+    const int col = tid % m;
+    const int r0 = tid / m;
+    const bool active = r0 < rstep;
+    if (active) {
+        const int64_t rowbase = (int64_t)perm[col] * n;
+        for (int r = r0; r < n; r += rstep)
+            At[(size_t)r * m + col] = R[(rowbase + r) * a.ordL + ord];
+    }
+    __syncthreads();
+
+    int rank = 0;
+    for (int c = 0; c < m && rank < n; ++c) {
+        // wave 0: pivot search among rows that are not pivots yet, snapshot of column c
+        if (tid < 64) {
+            int pr = -1;
+            for (int base = 0; base < n; base += 64) {
+                const int r = base + tid;
+                uint64_t v = 0;
+                if (r < n) {
+                    v = At[(size_t)r * m + c];
+                    fcol[r] = v;
+                }
+                const uint64_t mask = __ballot(r < n && v != 0 && !ispiv[r]);
+                if (mask && pr < 0)
+                    pr = base + __ffsll((unsigned long long)mask) - 1;
+            }
+            if (tid == 0) {
+                s_state[2 + (c & 1)] = pr;
+                if (pr >= 0)
+                    ispiv[pr] = 1;
+            }
+        }
+        __syncthreads();
+        const int pr = s_state[2 + (c & 1)];  // double-buffered: no second barrier needed
+        if (pr < 0)
+            continue;  // uniform
+        if (tid == 0) {
+            prow[rank] = pr;
+            pivcol[rank] = c;
+        }
+        const uint64_t pv = fcol[pr];
+        if (active) {
+            const uint64_t pcol = At[(size_t)pr * m + col];
+#pragma unroll 4
+            for (int r = r0; r < n; r += rstep) {
+                if (r == pr)
+                    continue;
+                uint64_t *x = At + (size_t)r * m + col;
+                *x = submod_g(mulmod_g(pv, *x, a.mc), mulmod_g(fcol[r], pcol, a.mc), p);
+            }
+        }
+        ++rank;
+        __syncthreads();
+    }
+    // pivots of the reduced echelon form: 1 / At[prow[k]][pivcol[k]]
+    for (int k = tid; k < rank; k += kStepThreads)
+        inv[k] = invmod_g(At[(size_t)prow[k] * m + pivcol[k]], a.mc);
+    if (tid == 0) {
+        // permutation list of left_nullspace_compact: pivots, then non-pivots
+        int q = 0, np = rank;
+        for (int i = 0; i < m; ++i) {
+            if (q < rank && pivcol[q] == i)
+                plist[q++] = i;
+            else
+                plist[np++] = i;
+        }
+    }
+    __syncthreads();
+    const int nullity = m - rank;
+    const int mode = nullity == 0 ? 2 : (nullity == m ? 0 : 1);
+    if (mode == 1) {
+        for (int idx = tid; idx < nullity * rank; idx += kStepThreads) {
+            const int t = idx / rank, r = idx - t * rank;
+            const uint64_t v = mulmod_g(At[(size_t)prow[r] * m + plist[rank + t]], inv[r], a.mc);
+            a.X[(size_t)t * m + r] = v ? p - v : 0;
+        }
+        for (int i = tid; i < m; i += kStepThreads) {
+            const int row = perm[plist[i]];
+            a.piv[i] = row;
+            a.rowinfo[row] = i < rank ? -1 - i : i - rank;
+        }
+        for (int i = tid; i < rank; i += kStepThreads)
+            a.shift[perm[plist[i]]] += 1;
+    } else if (mode == 2) {
+        for (int i = tid; i < m; i += kStepThreads)
+            a.shift[i] += a.ordL - ord;
+    }
+    if (tid == 0) {
+        int *c = a.ctl;
+        c[C_MODE] = mode;
+        c[C_RANK] = rank;
+        if (mode != 0)
+            c[C_FLIP] = 1;
+        if (mode == 2) {
+            c[C_SHIFT] = a.ordL - ord;
+            c[C_LOLD] = c[C_LEN];
+            c[C_LEN] += a.ordL - ord;
+            c[C_DONE] = 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernel 2: apply the step to appbas (and to the residual series)
+// ---------------------------------------------------------------------------
+// base + sum_r X[t][r] * src[row piv[r], column j, coefficient d]
+template <bool SMALL>
+__device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArgs &a, int t, int rank,
+                                              int stride, int cols, int j, int d, uint64_t base)
+{
+    const uint64_t *xr = a.X + (size_t)t * a.m;
+    if constexpr (SMALL) {
+        // p < 2^30: products < 2^60, 15 of them plus a residue fit in 64 bits
+        uint64_t acc = base;
+        int cnt = 0;
+        for (int r = 0; r < rank; ++r) {
+            acc += xr[r] * src[((size_t)a.piv[r] * cols + j) * stride + d];
+            if (++cnt == 15) {
+                acc = barrett(acc, a.mc.bm, a.mc.p);
+                cnt = 0;
+            }
+        }
+        return barrett(acc, a.mc.bm, a.mc.p);
+    } else {
+        uint64_t hi = 0, lo = base;
+        int cnt = 0;
+        for (int r = 0; r < rank; ++r) {
+            mac128(hi, lo, xr[r], src[((size_t)a.piv[r] * cols + j) * stride + d]);
+            if (++cnt == 15) {
+                lo = reduce128(hi, lo, a.mc);
+                hi = 0;
+                cnt = 0;
+            }
+        }
+        return reduce128(hi, lo, a.mc);
+    }
+}
+
+template <bool SMALL>
+__global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
+{
+    const int mode = a.ctl[C_MODE];
+    if (mode == 0)
+        return;
+    const int cur = a.ctl[C_CUR];
+    const int m = a.m, n = a.n, cap = a.cap;
+    const uint64_t *src = a.app[cur];
+    uint64_t *dst = a.app[cur ^ 1];
+    const unsigned tstride = gridDim.x * blockDim.x;
+    const unsigned t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 2) {
+        const int s = a.ctl[C_SHIFT], Lold = a.ctl[C_LOLD];
+        const unsigned Lnew = (unsigned)(Lold + s);
+        const unsigned items = (unsigned)(m * m) * Lnew;
+        for (unsigned idx = t0; idx < items; idx += tstride) {
+            const unsigned e = idx / Lnew;
+            const int d = (int)(idx - e * Lnew);
+            dst[(size_t)e * cap + d] = d >= s ? src[(size_t)e * cap + d - s] : 0;
+        }
+        return;
+    }
+    const int rank = a.ctl[C_RANK];
+    const int L = a.ctl[C_LEN];
+    const unsigned L1 = (unsigned)(L + 1 <= cap ? L + 1 : cap);
+    const unsigned items = (unsigned)(m * m) * L1;
+    for (unsigned idx = t0; idx < items; idx += tstride) {
+        const unsigned e = idx / L1;
+        const int d = (int)(idx - e * L1);
+        const int i = (int)(e / (unsigned)m), j = (int)(e - (unsigned)i * m);
+        const int ri = a.rowinfo[i];
+        uint64_t v;
+        if (ri >= 0) {
+            v = d < L ? row_combo<SMALL>(src, a, ri, rank, cap, m, j, d, src[(size_t)e * cap + d]) : 0;
+        } else {
+            v = d >= 1 ? src[(size_t)e * cap + d - 1] : 0;
+            if (d == L && v != 0)
+                atomicOr(&a.ctl[C_GROW], 1);
+        }
+        dst[(size_t)e * cap + d] = v;
+    }
+    // residual series: coefficients ord+1 .. ordL-1
+    const uint64_t *rs = a.R[cur];
+    uint64_t *rd = a.R[cur ^ 1];
+    const unsigned span = (unsigned)(a.ordL - ord - 1);
+    if ((int)span <= 0)
+        return;
+    const unsigned ritems = (unsigned)(m * n) * span;
+    for (unsigned idx = t0; idx < ritems; idx += tstride) {
+        const unsigned e = idx / span;
+        const int d = ord + 1 + (int)(idx - e * span);
+        const int i = (int)(e / (unsigned)n), j = (int)(e - (unsigned)i * n);
+        const int ri = a.rowinfo[i];
+        rd[(size_t)e * a.ordL + d] =
+            ri >= 0 ? row_combo<SMALL>(rs, a, ri, rank, a.ordL, n, j, d, rs[(size_t)e * a.ordL + d])
+                    : rs[(size_t)e * a.ordL + d - 1];
+    }
+}
+
+// appbas <- identity (capacity cap), R <- F truncated to ordL, control reset
+__global__ void leaf_init_kernel(LeafArgs a, const uint64_t *__restrict__ F, int64_t strideF,
+                                 int64_t lenF)
+{
+    const int m = a.m, n = a.n, cap = a.cap;
+    const int64_t tstride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t itemsA = (int64_t)m * m * cap;
+    for (int64_t idx = t0; idx < itemsA; idx += tstride) {
+        const int64_t e = idx / cap;
+        const int d = (int)(idx - e * cap);
+        const int i = (int)(e / m), j = (int)(e - (int64_t)i * m);
+        a.app[0][idx] = (d == 0 && i == j) ? 1 : 0;
+    }
+    const int64_t itemsR = (int64_t)m * n * a.ordL;
+    for (int64_t idx = t0; idx < itemsR; idx += tstride) {
+        const int64_t e = idx / a.ordL;
+        const int d = (int)(idx - e * a.ordL);
+        a.R[0][idx] = d < lenF ? F[e * strideF + d] : 0;
+    }
+    if (t0 < C_WORDS)
+        a.ctl[t0] = (t0 == C_LEN) ? 1 : 0;
+}
+
+// copies an m x m x L block between two capacities (entry-major)
+__global__ void copy_poly_kernel(uint64_t *__restrict__ dst, int64_t dcap,
+                                 const uint64_t *__restrict__ src, int64_t scap, int64_t entries,
+                                 int64_t len)
+{
+    const int64_t items = entries * len;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < items;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = idx / len, d = idx - e * len;
+        dst[e * dcap + d] = src[e * scap + d];
+    }
+}
+
+unsigned grid_for(int64_t items)
+{
+    int64_t b = (items + 255) / 256;
+    if (b < 1)
+        b = 1;
+    if (b > 4096)
+        b = 4096;
+    return (unsigned)b;
+}
+
+// Bump allocator over one device buffer; results are moved down to the
+// caller's mark so that temporaries of a recursion level are reclaimed.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, top = 0;
+    uint64_t *alloc(size_t words)
+    {
+        size_t bytes = (words * 8 + 255) & ~(size_t)255;
+        if (top + bytes > size)
+            return nullptr;
+        uint64_t *p = reinterpret_cast<uint64_t *>(base + top);
+        top += bytes;
+        return p;
+    }
+};
+
+struct Poly {
+    uint64_t *d;
+    int64_t cap, len;
+};
+
+constexpr int kPmbasisThres = 32;  // PMBASIS_THRES, nmod_poly_mat_approximant.h:101
+
+// M-Basis of F (m x n, entries at F + e*strideF, length lenF) at order ordL;
+// the result is placed at the arena mark.
+pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, int64_t lenF,
+                 int m, int n, int ordL, int64_t *dShift, Poly *out, hipStream_t s)
+{
+    const size_t mark = ar.top;
+    const int cap = ordL + 1;
+    LeafArgs a{};
+    a.m = m;
+    a.n = n;
+    a.ordL = ordL;
+    a.cap = cap;
+    a.app[0] = ar.alloc((size_t)m * m * cap);
+    a.app[1] = ar.alloc((size_t)m * m * cap);
+    a.R[0] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
+    a.R[1] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
+    a.X = ar.alloc((size_t)m * m);
+    uint64_t *ints = ar.alloc((size_t)m + C_WORDS + 8);
+    if (!a.app[0] || !a.app[1] || !a.R[0] || !a.R[1] || !a.X || !ints)
+        return fail(PMLG_ERR_ALLOC, "pmbasis: device arena exhausted");
+    a.piv = reinterpret_cast<int *>(ints);
+    a.rowinfo = a.piv + m;
+    a.ctl = a.rowinfo + m;
+    a.shift = dShift;
+    a.mc = ctx->mc;
+    const int64_t lt = lenF < ordL ? lenF : ordL;
+    hipLaunchKernelGGL(leaf_init_kernel, dim3(grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL)),
+                       dim3(256), 0, s, a, F, strideF, lt);
+    HIP_TRY(hipGetLastError(), "leaf_init_kernel");
+    const size_t lds = ((size_t)n * m + 3 * kMaxDim) * 8 + 5 * kMaxDim * sizeof(int);
+    static bool attr = false;
+    if (!attr) {
+        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(((size_t)kMaxDim * kMaxDim + 3 * kMaxDim) * 8 +
+                                          5 * kMaxDim * sizeof(int))),
+                "hipFuncSetAttribute(mbasis_step_kernel)");
+        attr = true;
+    }
+    const unsigned ugrid = grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL);
+    const bool small = ctx->p < ((uint64_t)1 << 30);
+    for (int ord = 0; ord < ordL; ++ord) {
+        hipLaunchKernelGGL(mbasis_step_kernel, dim3(1), dim3(kStepThreads), lds, s, a, ord);
+        if (small)
+            hipLaunchKernelGGL(mbasis_update_kernel<true>, dim3(ugrid), dim3(256), 0, s, a, ord);
+        else
+            hipLaunchKernelGGL(mbasis_update_kernel<false>, dim3(ugrid), dim3(256), 0, s, a, ord);
+    }
+    HIP_TRY(hipGetLastError(), "mbasis kernels");
+    int ctl[C_WORDS];
+    HIP_TRY(hipMemcpyAsync(ctl, a.ctl, sizeof(ctl), hipMemcpyDeviceToHost, s), "D2H(ctl)");
+    HIP_TRY(hipStreamSynchronize(s), "leaf sync");
+    const int res = ctl[C_CUR] ^ ctl[C_FLIP];
+    const int L = ctl[C_LEN] + ctl[C_GROW];
+    // result to the mark (app[0] sits there; move app[1] down if needed)
+    out->d = a.app[0];
+    out->cap = cap;
+    out->len = L;
+    if (res == 1) {
+        hipLaunchKernelGGL(copy_poly_kernel, dim3(grid_for((int64_t)m * m * L)), dim3(256), 0, s,
+                           a.app[0], (int64_t)cap, a.app[1], (int64_t)cap, (int64_t)m * m, (int64_t)L);
+        HIP_TRY(hipGetLastError(), "copy_poly_kernel");
+    }
+    ar.top = mark + (((size_t)m * m * cap * 8 + 255) & ~(size_t)255);
+    return PMLG_OK;
+}
+
+// PM-Basis recursion (approximant_basis.c:33-69); result at the arena mark.
+pmlg_status pmb_rec(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, int64_t lenF,
+                    int m, int n, int64_t order, int64_t *dShift, int method, int thres, Poly *out,
+                    hipStream_t s)
+{
+    const int64_t lt = lenF < order ? lenF : order;
+    if (order <= thres)
+        return leaf(ctx, ar, F, strideF, lt, m, n, (int)order, dShift, out, s);
+    const size_t mark = ar.top;
+    const int64_t order1 = order >> 1, order2 = order - order1;
+    Poly P1;
+    pmlg_status st = pmb_rec(ctx, ar, F, strideF, lt, m, n, order1, dShift, method, thres, &P1, s);
+    if (st != PMLG_OK)
+        return st;
+    // residual = (P1 * F mod x^order) div x^order1
+    uint64_t *Rz = ar.alloc((size_t)m * n * order2);
+    if (!Rz)
+        return fail(PMLG_ERR_ALLOC, "pmbasis: device arena exhausted");
+    st = mulmid_dev_strided(ctx, Rz, order2, P1.d, P1.cap, F, strideF, m, m, n, P1.len, lt, order1,
+                            order, method, s);
+    if (st != PMLG_OK)
+        return st;
+    Poly P2;
+    st = pmb_rec(ctx, ar, Rz, order2, order2, m, n, order2, dShift, method, thres, &P2, s);
+    if (st != PMLG_OK)
+        return st;
+    // appbas = P2 * P1
+    const int64_t L = P1.len + P2.len - 1;
+    uint64_t *P = ar.alloc((size_t)m * m * L);
+    if (!P)
+        return fail(PMLG_ERR_ALLOC, "pmbasis: device arena exhausted");
+    st = mulmid_dev_strided(ctx, P, L, P2.d, P2.cap, P1.d, P1.cap, m, m, m, P2.len, P1.len, 0, L,
+                            method, s);
+    if (st != PMLG_OK)
+        return st;
+    // move the result down to the mark (P1, Rz and P2 are dead)
+    uint64_t *dst = reinterpret_cast<uint64_t *>(ar.base + mark);
+    hipLaunchKernelGGL(copy_poly_kernel, dim3(grid_for((int64_t)m * m * L)), dim3(256), 0, s, dst, L,
+                       P, L, (int64_t)m * m, L);
+    HIP_TRY(hipGetLastError(), "copy_poly_kernel");
+    ar.top = mark + (((size_t)m * m * L * 8 + 255) & ~(size_t)255);
+    out->d = dst;
+    out->cap = L;
+    out->len = L;
+    return PMLG_OK;
+}
+
+size_t arena_bytes(int64_t m, int64_t n, int64_t order)
+{
+    // generous bound: a few copies of every level's operands
+    const double lvl = (double)m * m * (order + 2) * 4 + (double)m * n * (order + 1) * 2;
+    const double leafb = (double)m * m * (kPmbasisThres + 2) * 2 + (double)m * n * 34 * 2 + m * m + 64;
+    return (size_t)((2.5 * lvl + leafb + 4096.0) * 8.0);
+}
+
+pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP,
+                         int64_t *shift, const uint64_t *dF, int64_t m, int64_t n, int64_t lenF,
+                         int64_t order, int method, bool iterative, hipStream_t s,
+                         uint64_t *hP)
+{
+    if (m <= 0 || n <= 0 || m > kMaxDim || n > kMaxDim || order < 0 || lenF < 0)
+        return fail(PMLG_ERR_ARG, "approximant basis: dimensions must be in [1, 128], order >= 0");
+    if (capP < order + 1)
+        return fail(PMLG_ERR_ARG, "approximant basis: output capacity must be at least order + 1");
+    // device arena
+    const int64_t ordleaf = iterative ? order : std::min<int64_t>(order, kPmbasisThres);
+    size_t need = arena_bytes(m, n, order) +
+                  (size_t)(m * m * (ordleaf + 2) * 2 + m * n * (ordleaf + 1) * 2) * 8;
+    HIP_TRY(ctx->arena.ensure(need), "hipMalloc(arena)");
+    Arena ar;
+    ar.base = static_cast<char *>(ctx->arena.ptr);
+    ar.size = ctx->arena.bytes;
+    int64_t *dShift = reinterpret_cast<int64_t *>(ar.alloc((size_t)m));
+    HIP_TRY(hipMemcpyAsync(dShift, shift, (size_t)m * 8, hipMemcpyHostToDevice, s), "H2D(shift)");
+    Poly P;
+    pmlg_status st;
+    if (iterative)
+        st = leaf(ctx, ar, dF, lenF, std::min(lenF, order), (int)m, (int)n, (int)order, dShift, &P, s);
+    else
+        st = pmb_rec(ctx, ar, dF, lenF, lenF, (int)m, (int)n, order, dShift, method, kPmbasisThres,
+                     &P, s);
+    if (st != PMLG_OK)
+        return st;
+    HIP_TRY(hipMemcpyAsync(shift, dShift, (size_t)m * 8, hipMemcpyDeviceToHost, s), "D2H(shift)");
+    const int64_t L = std::min<int64_t>(P.len, capP);
+    if (hP) {
+        std::vector<uint64_t> tmp((size_t)(m * m * P.cap));
+        HIP_TRY(hipMemcpyAsync(tmp.data(), P.d, tmp.size() * 8, hipMemcpyDeviceToHost, s), "D2H(P)");
+        HIP_TRY(hipStreamSynchronize(s), "sync");
+        // normalised length (the basis has degree at most order)
+        int64_t Ln = 0;
+        for (int64_t e = 0; e < m * m; ++e)
+            for (int64_t d = L - 1; d >= Ln; --d)
+                if (tmp[(size_t)(e * P.cap + d)]) {
+                    Ln = d + 1;
+                    break;
+                }
+        for (int64_t e = 0; e < m * m; ++e) {
+            std::memcpy(hP + e * capP, tmp.data() + e * P.cap, (size_t)Ln * 8);
+            std::memset(hP + e * capP + Ln, 0, (size_t)(capP - Ln) * 8);
+        }
+        *lenP = Ln;
+    } else {
+        hipLaunchKernelGGL(copy_poly_kernel, dim3(grid_for(m * m * L)), dim3(256), 0, s, dP, capP, P.d,
+                           P.cap, (int64_t)(m * m), L);
+        HIP_TRY(hipGetLastError(), "copy_poly_kernel");
+        HIP_TRY(hipStreamSynchronize(s), "sync");
+        *lenP = L;
+    }
+    return PMLG_OK;
+}
+
+}  // namespace
+
+}  // namespace pmlg
+
+using namespace pmlg;
+
+extern "C" {
+
+pmlg_status pmlg_pmbasis_dev(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP,
+                             int64_t *shift, const uint64_t *dF, int64_t m, int64_t n,
+                             int64_t lenF, int64_t order, int method, void *stream)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    return basis_common(ctx, dP, capP, lenP, shift, dF, m, n, lenF, order, method, false, s, nullptr);
+}
+
+static pmlg_status basis_host(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
+                              const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
+                              int64_t capP, int method, bool iterative)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (m <= 0 || n <= 0 || lenF < 0 || !P || !lenP || !shift || (lenF > 0 && !F))
+        return fail(PMLG_ERR_ARG, "approximant basis: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t fb = (size_t)(m * n * (lenF > 0 ? lenF : 1)) * 8;
+    HIP_TRY(ctx->in_a.ensure(fb), "hipMalloc(F)");
+    if (lenF > 0)
+        HIP_TRY(hipMemcpyAsync(ctx->in_a.ptr, F, fb, hipMemcpyHostToDevice, s), "H2D(F)");
+    return basis_common(ctx, nullptr, capP, lenP, shift, static_cast<const uint64_t *>(ctx->in_a.ptr),
+                        m, n, lenF, order, method, iterative, s, P);
+}
+
+pmlg_status pmlg_pmbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
+                         const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
+                         int64_t capP, int method)
+{
+    return basis_host(ctx, P, lenP, shift, F, m, n, lenF, order, capP, method, false);
+}
+
+pmlg_status pmlg_mbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
+                        const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
+                        int64_t capP)
+{
+    return basis_host(ctx, P, lenP, shift, F, m, n, lenF, order, capP, PMLG_METHOD_AUTO, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+"""GPU parity of approximant bases (PM-Basis, M-Basis) with the oracle.
+
+Follows the shape of the reference tests (flint-extras/src/nmod_poly_mat_extra/
+test/t-pmbasis.c:70-109 and testing_collection.h: random primes, dimensions,
+orders, the eight shift families, zero and rank-deficient inputs), but checks
+bit-exact equality with the oracle's restatement of the reference algorithm,
+on top of the certificate the reference uses.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P60 = 1152921504556515329
+P30 = 998244353
+PRIMES_ANY = [3, 5, 521, 524309, 536870923, 549755813911, 576460752303423619]  # order <= 32: leaf only
+PRIMES_FFT = [P30, P60, 7340033, 469762049]
+
+
+@pytest.fixture(scope="module")
+def pml():
+    import pml_amd
+    return pml_amd
+
+
+_ctxs = {}
+
+
+def ctx_for(pml, p):
+    if p not in _ctxs:
+        _ctxs[p] = pml.Context(p)
+    return _ctxs[p]
+
+
+SHIFTS = {
+    "uniform": lambda m, d: np.zeros(m),
+    "increasing": lambda m, d: np.arange(m),
+    "decreasing": lambda m, d: m - np.arange(m),
+    "shuffle": lambda m, d: np.random.default_rng(m).permutation(m),
+    "hermite": lambda m, d: np.arange(m) * m * d,
+    "rhermite": lambda m, d: (m - np.arange(m)) * m * d,
+    "plateau": lambda m, d: np.where(np.arange(m) < m // 2, 0, m * d),
+    "rplateau": lambda m, d: np.where(np.arange(m) < m // 2, m * d, 0),
+}
+
+
+def make(rng, kind, m, n, ln, p):
+    if kind == "uniform":
+        return rng.integers(0, p, (m, n, ln), dtype=np.uint64)
+    if kind == "zero":
+        return np.zeros((m, n, ln), dtype=np.uint64)
+    if kind == "sparse":
+        F = rng.integers(0, p, (m, n, ln), dtype=np.uint64)
+        return F * (rng.random((m, n, ln)) < 0.05)
+    if kind == "rkdef":  # rank at most 1 at every coefficient
+        u = rng.integers(0, p, (m, 1, ln), dtype=np.uint64)
+        v = rng.integers(0, min(p, 4), (1, n, 1), dtype=np.uint64)
+        return (u.astype(object) * v.astype(object) % p).astype(np.uint64)
+    if kind == "cdeg":  # unbalanced column degrees
+        F = rng.integers(0, p, (m, n, ln), dtype=np.uint64)
+        for j in range(n):
+            F[:, j, max(1, ln * (j + 1) // (n + 1)):] = 0
+        return F
+
+
+def check(pml, p, F, order, shift, fn="pmbasis"):
+    P_ref, s_ref = getattr(O, fn)(F, order, shift, p)
+    P_dev, s_dev = getattr(ctx_for(pml, p), fn)(F, order, shift)
+    assert np.array_equal(O.normalise(P_dev), O.normalise(P_ref))
+    assert np.array_equal(s_dev, s_ref)
+    return P_dev
+
+
+@pytest.mark.parametrize("shift_name", sorted(SHIFTS))
+def test_pmbasis_shift_families(pml, shift_name):
+    rng = np.random.default_rng(31)
+    for it in range(6):
+        p = PRIMES_FFT[it % len(PRIMES_FFT)]
+        m, n = int(rng.integers(1, 10)), int(rng.integers(1, 8))
+        order = int(rng.integers(1, 150))
+        F = make(rng, "uniform", m, n, int(rng.integers(0, 150)), p)
+        shift = SHIFTS[shift_name](m, order).astype(np.int64)
+        P = check(pml, p, F, order, shift)
+        assert O.is_approximant_basis(P, F, order, shift, p) == 0
+
+
+@pytest.mark.parametrize("kind", ["zero", "sparse", "rkdef", "cdeg"])
+def test_pmbasis_matrix_families(pml, kind):
+    rng = np.random.default_rng(37)
+    for it in range(4):
+        p = PRIMES_FFT[it % len(PRIMES_FFT)]
+        m, n = int(rng.integers(2, 9)), int(rng.integers(1, 6))
+        order = int(rng.integers(1, 120))
+        F = make(rng, kind, m, n, order, p)
+        check(pml, p, F, order, rng.integers(0, 4, m).astype(np.int64))
+
+
+@pytest.mark.parametrize("p", PRIMES_ANY)
+def test_leaf_any_prime(pml, p):
+    """Orders up to 32 run only the device M-Basis step (no transform), so
+    every word-size prime below 2^62 is allowed, as in the reference tests."""
+    rng = np.random.default_rng(p % 9973)
+    for (m, n, order) in [(4, 2, 32), (5, 5, 20), (3, 1, 31), (9, 4, 17)]:
+        F = make(rng, "uniform", m, n, order + 3, p)
+        sh = rng.integers(0, 3, m).astype(np.int64)
+        check(pml, p, F, order, sh, "pmbasis")
+        check(pml, p, F, order, sh, "mbasis")
+
+
+def test_mbasis_long_order(pml):
+    rng = np.random.default_rng(41)
+    for p in (P30, P60):
+        F = make(rng, "uniform", 6, 2, 90, p)
+        check(pml, p, F, 90, np.zeros(6, dtype=np.int64), "mbasis")
+
+
+def test_full_rank_residual_stops(pml):
+    """n >= m: the residual reaches full row rank, so the basis is X^k times
+    the current one and the loop stops (nmod_mat_poly_mbasis.c:221-231)."""
+    rng = np.random.default_rng(43)
+    for p in (P30, P60):
+        for (m, n, order) in [(3, 5, 40), (4, 4, 70), (1, 3, 50)]:
+            F = make(rng, "uniform", m, n, order, p)
+            check(pml, p, F, order, np.zeros(m, dtype=np.int64))
+
+
+def test_pmbasis_dims_64(pml):
+    rng = np.random.default_rng(47)
+    F = make(rng, "uniform", 64, 32, 200, P30)
+    P = check(pml, P30, F, 200, np.zeros(64, dtype=np.int64))
+    assert O.is_approximant_basis(P, F, 200, np.zeros(64, dtype=np.int64), P30) == 0

@@ -306,3 +306,33 @@ def test_mbasis_equals_pmbasis_generic():
             P2, s2 = O.pmbasis(F, order, sh, p)
             assert np.array_equal(O.normalise(P1), O.normalise(P2))
             assert np.array_equal(s1, s2)
+
+
+def test_golden_fixture_small_entries():
+    """The committed fixture agrees with the oracle on the entries that are
+    cheap to recompute (the others are checked on the GPU at full size)."""
+    import hashlib
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+    e = g["c1_mul_4x4x4_len256_p60"]
+    A = O.rand_uniform((4, 4, 256), e["p"], e["seedA"])
+    B = O.rand_uniform((4, 4, 256), e["p"], e["seedB"])
+    C = O.polmat_mul(A, B, e["p"], "ntt")
+    assert hashlib.sha256(C.tobytes()).hexdigest() == e["sha256"]
+    assert g["c3_pmbasis_64x32_order2048_p30"]["certificate"] == 0
+
+
+def test_rand_uniform_stream():
+    """SplitMix64 stream (fixed constants) and rejection to [0, p)."""
+    x = O.rand_uniform((3,), 2**61, 7)
+    st, vals = 7, []
+    for _ in range(3):
+        st = (st + 0x9E3779B97F4A7C15) % 2**64
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) % 2**64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) % 2**64
+        vals.append((z ^ (z >> 31)) >> 3)
+    assert [int(v) for v in x] == vals
+    y = O.rand_uniform((1000,), 998244353, 1)
+    assert int(y.max()) < 998244353
