@@ -108,6 +108,16 @@ pmlg_status pmlg_pmbasis_dev(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *
                              int64_t *shift, const uint64_t *dF, int64_t m, int64_t n,
                              int64_t lenF, int64_t order, int method, void *stream);
 
+/* Rank (and determinant when r == c, else 0) of an r x c constant matrix
+ * (row-major, host pointer, r*c <= 2^22), by Gaussian elimination on the
+ * device.  Used by the approximant-basis certificate; no reference counterpart
+ * (FLINT's nmod_mat_rank / nmod_mat_det there). */
+pmlg_status pmlg_const_rank_det(pmlg_ctx ctx, const uint64_t *M, int64_t r, int64_t c,
+                                int64_t *rank, uint64_t *det);
+/* out[i][j] = A[i][j](x) for an r x c polynomial matrix (host pointers). */
+pmlg_status pmlg_polmat_eval(pmlg_ctx ctx, uint64_t *out, const uint64_t *A, int64_t r, int64_t c,
+                             int64_t len, uint64_t x);
+
 /* Per-stage device timing (HIP events around each launch).  Stages:
  * 0 forward NTT, 1 pointwise product, 2 inverse NTT (+ merge). */
 void pmlg_profile_enable(pmlg_ctx ctx, int on);

@@ -1,0 +1,600 @@
+/*
+ * FLINT-shaped drop-in entry points of libpmlgpu (declared in
+ * include/pmlgpu_flint.h): PML's nmod_poly_mat_multiply / mulmid /
+ * pmbasis / mbasis / is_approximant_basis, computed on the GPU.
+ *
+ * Each call packs the FLINT polynomial matrices into entry-major buffers
+ * (entry (i,j) coefficient d at [(i*cols + j)*len + d]), runs the device
+ * routine on a context for the matrices' modulus, and unpacks the result into
+ * the caller's matrix, growing and normalising its entries.  Contexts are
+ * cached per modulus for the life of the process; a context serialises its own
+ * calls, so several threads may use the drop-in at once.  The device is
+ * PMLG_DEVICE (default 0).  Failures abort with a message, as flint_throw
+ * does: the drop-in has no CPU fallback.
+ *
+ * Entry growth uses FLINT's nmod_poly_fit_length when the process has it
+ * (the drop-in then lives beside a real FLINT, whose memory functions own the
+ * entries), else the pmlg_nmod_poly_* functions defined here.
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/pmlgpu_flint.h"
+
+/* ------------------------------------------------------------------------ */
+/* memory management of the FLINT subset                                     */
+/* ------------------------------------------------------------------------ */
+
+static void nmod_init_fields(nmod_t *mod, ulong n)
+{
+    /* FLINT's nmod_t: modulus, normalisation shift and the 2-by-1 reciprocal
+     * of the normalised modulus, floor((2^128 - 1) / (n << norm)) - 2^64. */
+    mod->n = n;
+    mod->norm = n ? (ulong)__builtin_clzl(n) : 0;
+    if (n) {
+        const unsigned __int128 d = (unsigned __int128)(n << mod->norm);
+        mod->ninv = (ulong)((~(unsigned __int128)0) / d - ((unsigned __int128)1 << 64));
+    } else {
+        mod->ninv = 0;
+    }
+}
+
+void pmlg_nmod_poly_init(nmod_poly_t poly, ulong n)
+{
+    poly->coeffs = NULL;
+    poly->alloc = 0;
+    poly->length = 0;
+    nmod_init_fields(&poly->mod, n);
+}
+
+void pmlg_nmod_poly_clear(nmod_poly_t poly)
+{
+    free(poly->coeffs);
+    poly->coeffs = NULL;
+    poly->alloc = 0;
+    poly->length = 0;
+}
+
+void pmlg_nmod_poly_fit_length(nmod_poly_t poly, slong len)
+{
+    if (len <= poly->alloc)
+        return;
+    slong want = len > 2 * poly->alloc ? len : 2 * poly->alloc;
+    ulong *c = (ulong *)realloc(poly->coeffs, (size_t)want * sizeof(ulong));
+    if (!c) {
+        fprintf(stderr, "pmlgpu: out of host memory (%ld coefficients)\n", (long)want);
+        abort();
+    }
+    poly->coeffs = c;
+    poly->alloc = want;
+}
+
+static void poly_normalise(nmod_poly_struct *poly)
+{
+    while (poly->length > 0 && poly->coeffs[poly->length - 1] == 0)
+        poly->length--;
+}
+
+void pmlg_nmod_poly_set_coeff_ui(nmod_poly_t poly, slong j, ulong c)
+{
+    c %= poly->mod.n;
+    if (j >= poly->length) {
+        if (c == 0)
+            return;
+        pmlg_nmod_poly_fit_length(poly, j + 1);
+        memset(poly->coeffs + poly->length, 0, (size_t)(j - poly->length) * sizeof(ulong));
+        poly->length = j + 1;
+    }
+    poly->coeffs[j] = c;
+    if (j == poly->length - 1)
+        poly_normalise(poly);
+}
+
+ulong pmlg_nmod_poly_get_coeff_ui(const nmod_poly_t poly, slong j)
+{
+    return (j >= 0 && j < poly->length) ? poly->coeffs[j] : 0;
+}
+
+void pmlg_nmod_poly_mat_init(nmod_poly_mat_t mat, slong rows, slong cols, ulong n)
+{
+    mat->entries = NULL;
+    mat->rows = NULL;
+    mat->r = rows;
+    mat->c = cols;
+    mat->modulus = n;
+    if (rows > 0 && cols > 0) {
+        mat->entries = (nmod_poly_struct *)malloc((size_t)(rows * cols) * sizeof(nmod_poly_struct));
+        if (!mat->entries)
+            abort();
+        for (slong e = 0; e < rows * cols; e++)
+            pmlg_nmod_poly_init(mat->entries + e, n);
+    }
+    if (rows > 0) {
+        mat->rows = (nmod_poly_struct **)malloc((size_t)rows * sizeof(nmod_poly_struct *));
+        if (!mat->rows)
+            abort();
+        for (slong i = 0; i < rows; i++)
+            mat->rows[i] = mat->entries ? mat->entries + i * cols : NULL;
+    }
+}
+
+void pmlg_nmod_poly_mat_clear(nmod_poly_mat_t mat)
+{
+    if (mat->entries)
+        for (slong e = 0; e < mat->r * mat->c; e++)
+            pmlg_nmod_poly_clear(mat->entries + e);
+    free(mat->entries);
+    free(mat->rows);
+    mat->entries = NULL;
+    mat->rows = NULL;
+}
+
+void pmlg_nmod_poly_mat_zero(nmod_poly_mat_t mat)
+{
+    for (slong i = 0; i < mat->r; i++)
+        for (slong j = 0; j < mat->c; j++)
+            nmod_poly_mat_entry(mat, i, j)->length = 0;
+}
+
+slong pmlg_nmod_poly_mat_max_length(const nmod_poly_mat_t mat)
+{
+    slong len = 0;
+    for (slong i = 0; i < mat->r; i++)
+        for (slong j = 0; j < mat->c; j++) {
+            const slong l = nmod_poly_mat_entry(mat, i, j)->length;
+            len = l > len ? l : len;
+        }
+    return len;
+}
+
+int pmlg_nmod_poly_mat_equal(const nmod_poly_mat_t a, const nmod_poly_mat_t b)
+{
+    if (a->r != b->r || a->c != b->c)
+        return 0;
+    for (slong i = 0; i < a->r; i++)
+        for (slong j = 0; j < a->c; j++) {
+            const nmod_poly_struct *x = nmod_poly_mat_entry(a, i, j);
+            const nmod_poly_struct *y = nmod_poly_mat_entry(b, i, j);
+            if (x->length != y->length ||
+                (x->length && memcmp(x->coeffs, y->coeffs, (size_t)x->length * sizeof(ulong))))
+                return 0;
+        }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* contexts, errors, packing                                                 */
+/* ------------------------------------------------------------------------ */
+
+#define CTX_CACHE 32
+
+static pthread_mutex_t g_cache_mu = PTHREAD_MUTEX_INITIALIZER;
+static struct {
+    uint64_t p;
+    pmlg_ctx ctx;
+} g_cache[CTX_CACHE];
+static int g_cache_n = 0;
+
+typedef void (*fit_length_fn)(nmod_poly_struct *, slong);
+static fit_length_fn g_fit = NULL;
+static pthread_once_t g_fit_once = PTHREAD_ONCE_INIT;
+
+static void resolve_fit(void)
+{
+    /* a FLINT loaded in the process owns the entries' memory */
+    void *f = dlsym(RTLD_DEFAULT, "nmod_poly_fit_length");
+    g_fit = f ? (fit_length_fn)f : (fit_length_fn)pmlg_nmod_poly_fit_length;
+}
+
+static void die(const char *fn, const char *what)
+{
+    fprintf(stderr, "Exception (%s). %s\n", fn, what);
+    fflush(stderr);
+    abort();
+}
+
+static void check(pmlg_status st, const char *fn)
+{
+    if (st != PMLG_OK)
+        die(fn, pmlg_last_error());
+}
+
+static void destroy_cache(void)
+{
+    for (int i = 0; i < g_cache_n; i++)
+        pmlg_ctx_destroy(g_cache[i].ctx);
+    g_cache_n = 0;
+}
+
+static pmlg_ctx ctx_for(uint64_t p, const char *fn)
+{
+    pthread_mutex_lock(&g_cache_mu);
+    for (int i = 0; i < g_cache_n; i++)
+        if (g_cache[i].p == p) {
+            pmlg_ctx c = g_cache[i].ctx;
+            pthread_mutex_unlock(&g_cache_mu);
+            return c;
+        }
+    if (g_cache_n == CTX_CACHE) {
+        pthread_mutex_unlock(&g_cache_mu);
+        die(fn, "more than 32 distinct moduli in one process");
+    }
+    const char *dev = getenv("PMLG_DEVICE");
+    pmlg_ctx c = NULL;
+    pmlg_status st = pmlg_ctx_create(&c, p, dev ? atoi(dev) : 0);
+    if (st != PMLG_OK) {
+        pthread_mutex_unlock(&g_cache_mu);
+        die(fn, pmlg_last_error());
+    }
+    if (g_cache_n == 0)
+        atexit(destroy_cache);
+    g_cache[g_cache_n].p = p;
+    g_cache[g_cache_n].ctx = c;
+    g_cache_n++;
+    pthread_mutex_unlock(&g_cache_mu);
+    return c;
+}
+
+static uint64_t *xalloc(size_t words, const char *fn)
+{
+    uint64_t *b = (uint64_t *)malloc((words ? words : 1) * sizeof(uint64_t));
+    if (!b)
+        die(fn, "out of host memory");
+    return b;
+}
+
+/* entry-major copy of the first `len` coefficients of each entry, zero padded */
+static uint64_t *pack(const nmod_poly_mat_struct *M, slong len, const char *fn)
+{
+    const slong r = M->r, c = M->c;
+    uint64_t *b = xalloc((size_t)(r * c * len), fn);
+    for (slong i = 0; i < r; i++)
+        for (slong j = 0; j < c; j++) {
+            const nmod_poly_struct *e = nmod_poly_mat_entry(M, i, j);
+            uint64_t *d = b + (i * c + j) * len;
+            const slong l = e->length < len ? e->length : len;
+            if (l > 0)
+                memcpy(d, e->coeffs, (size_t)l * sizeof(uint64_t));
+            if (l < len)
+                memset(d + (l > 0 ? l : 0), 0, (size_t)(len - (l > 0 ? l : 0)) * sizeof(uint64_t));
+        }
+    return b;
+}
+
+/* res (r x c, entries of padded length len in buf) <- buf, normalised */
+static void unpack(nmod_poly_mat_struct *res, const uint64_t *buf, slong len)
+{
+    pthread_once(&g_fit_once, resolve_fit);
+    const slong r = res->r, c = res->c;
+    for (slong i = 0; i < r; i++)
+        for (slong j = 0; j < c; j++) {
+            nmod_poly_struct *e = nmod_poly_mat_entry(res, i, j);
+            const uint64_t *s = buf + (i * c + j) * len;
+            slong l = len;
+            while (l > 0 && s[l - 1] == 0)
+                l--;
+            if (l > 0) {
+                g_fit(e, l);
+                memcpy(e->coeffs, s, (size_t)l * sizeof(uint64_t));
+            }
+            e->length = l;
+        }
+}
+
+static void need_dims(int ok, const char *fn)
+{
+    if (!ok)
+        die(fn, "incompatible matrix dimensions");
+}
+
+/* ------------------------------------------------------------------------ */
+/* products                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* nmod_poly_mat_multiply (mul.c:19-97): the product is unique, so the
+ * reference's choice among its evaluation back-ends has no counterpart here;
+ * the device picks the pointwise kernel (int8-limb MFMA or plain) itself. */
+static void mul_impl(nmod_poly_mat_struct *res, const nmod_poly_mat_struct *A,
+                     const nmod_poly_mat_struct *B, const char *fn)
+{
+    need_dims(A->c == B->r && res->r == A->r && res->c == B->c, fn);
+    const slong lenA = pmlg_nmod_poly_mat_max_length(A);
+    const slong lenB = pmlg_nmod_poly_mat_max_length(B);
+    if (lenA == 0 || lenB == 0 || A->r == 0 || B->c == 0) {
+        pmlg_nmod_poly_mat_zero(res);
+        return;
+    }
+    const slong m = A->r, k = A->c, n = B->c, lenC = lenA + lenB - 1;
+    uint64_t *a = pack(A, lenA, fn);
+    uint64_t *b = pack(B, lenB, fn);
+    uint64_t *c = xalloc((size_t)(m * n * lenC), fn);
+    check(pmlg_polmat_mul(ctx_for(A->modulus, fn), c, a, b, m, k, n, lenA, lenB, PMLG_METHOD_AUTO),
+          fn);
+    free(a);
+    free(b);
+    unpack(res, c, lenC); /* A and B are packed already: res may alias them */
+    free(c);
+}
+
+void nmod_poly_mat_multiply(nmod_poly_mat_t res, const nmod_poly_mat_t A, const nmod_poly_mat_t B)
+{
+    mul_impl(res, A, B, "nmod_poly_mat_multiply");
+}
+
+void nmod_poly_mat_mul_geometric(nmod_poly_mat_t res, const nmod_poly_mat_t A,
+                                 const nmod_poly_mat_t B)
+{
+    mul_impl(res, A, B, "nmod_poly_mat_mul_geometric");
+}
+
+void nmod_poly_mat_mul_vandermonde1(nmod_poly_mat_t res, const nmod_poly_mat_t A,
+                                    const nmod_poly_mat_t B)
+{
+    mul_impl(res, A, B, "nmod_poly_mat_mul_vandermonde1");
+}
+
+void nmod_poly_mat_mul_vandermonde2(nmod_poly_mat_t res, const nmod_poly_mat_t A,
+                                    const nmod_poly_mat_t B)
+{
+    mul_impl(res, A, B, "nmod_poly_mat_mul_vandermonde2");
+}
+
+void nmod_poly_mat_mul_waksman(nmod_poly_mat_t res, const nmod_poly_mat_t A,
+                               const nmod_poly_mat_t B)
+{
+    mul_impl(res, A, B, "nmod_poly_mat_mul_waksman");
+}
+
+/* res = (A*B mod x^nhi) div x^nlo, from the first len1 / len2 coefficients of
+ * A / B (mulmid.c:28-42 and 470-497: the naive form multiplies in full, so
+ * entries are expected to be no longer than len1 / len2). */
+static void mulmid_impl(nmod_poly_mat_struct *res, const nmod_poly_mat_struct *A, slong len1,
+                        const nmod_poly_mat_struct *B, slong len2, slong nlo, slong nhi,
+                        const char *fn)
+{
+    need_dims(A->c == B->r && res->r == A->r && res->c == B->c, fn);
+    if (nlo < 0 || nhi < 0)
+        die(fn, "negative coefficient bound");
+    if (len1 == 0 || len2 == 0 || nlo >= nhi || A->r == 0 || B->c == 0) {
+        pmlg_nmod_poly_mat_zero(res);
+        return;
+    }
+    const slong m = A->r, k = A->c, n = B->c, lenR = nhi - nlo;
+    uint64_t *a = pack(A, len1, fn);
+    uint64_t *b = pack(B, len2, fn);
+    uint64_t *r = xalloc((size_t)(m * n * lenR), fn);
+    check(pmlg_polmat_mulmid(ctx_for(A->modulus, fn), r, a, b, m, k, n, len1, len2, nlo, nhi,
+                             PMLG_METHOD_AUTO),
+          fn);
+    free(a);
+    free(b);
+    unpack(res, r, lenR);
+    free(r);
+}
+
+void _nmod_poly_mat_mulmid(nmod_poly_mat_t res, const nmod_poly_mat_t A, slong len1,
+                           const nmod_poly_mat_t B, slong len2, slong nlo, slong nhi)
+{
+    const slong l1 = pmlg_nmod_poly_mat_max_length(A), l2 = pmlg_nmod_poly_mat_max_length(B);
+    mulmid_impl(res, A, len1 < l1 ? len1 : l1, B, len2 < l2 ? len2 : l2, nlo, nhi,
+                "_nmod_poly_mat_mulmid");
+}
+
+/* mulmid.c:499-559: nhi is clipped to the product length; the reference's
+ * truncations and shifts of the operands only save work, the device middle
+ * product sizes its transform from (lenA, lenB, nlo, nhi) directly. */
+void nmod_poly_mat_mulmid(nmod_poly_mat_t res, const nmod_poly_mat_t A, const nmod_poly_mat_t B,
+                          slong nlo, slong nhi)
+{
+    const slong len1 = pmlg_nmod_poly_mat_max_length(A), len2 = pmlg_nmod_poly_mat_max_length(B);
+    if (len1 > 0 && len2 > 0 && nhi > len1 + len2 - 1)
+        nhi = len1 + len2 - 1;
+    /* coefficients of degree >= nhi cannot reach the output */
+    mulmid_impl(res, A, len1 < nhi ? len1 : nhi, B, len2 < nhi ? len2 : nhi, nlo, nhi,
+                "nmod_poly_mat_mulmid");
+}
+
+/* ------------------------------------------------------------------------ */
+/* approximant bases                                                          */
+/* ------------------------------------------------------------------------ */
+
+static void basis_impl(nmod_poly_mat_struct *appbas, slong *shift, const nmod_poly_mat_struct *F,
+                       slong order, int iterative, const char *fn)
+{
+    const slong m = F->r, n = F->c;
+    need_dims(appbas->r == m && appbas->c == m, fn);
+    if (order < 0)
+        die(fn, "negative order");
+    /* only coefficients below the order matter (approximant_basis.c:41-42) */
+    slong lenF = pmlg_nmod_poly_mat_max_length(F);
+    lenF = lenF < order ? lenF : order;
+    uint64_t *f = pack(F, lenF, fn);
+    const slong cap = order + 1;
+    uint64_t *P = xalloc((size_t)(m * m * cap), fn);
+    int64_t *sh = (int64_t *)malloc((size_t)(m > 0 ? m : 1) * sizeof(int64_t));
+    if (!sh)
+        die(fn, "out of host memory");
+    for (slong i = 0; i < m; i++)
+        sh[i] = shift[i];
+    int64_t lenP = 0;
+    pmlg_ctx ctx = ctx_for(F->modulus, fn);
+    if (iterative)
+        check(pmlg_mbasis(ctx, P, &lenP, sh, f, m, n, lenF, order, cap), fn);
+    else
+        check(pmlg_pmbasis(ctx, P, &lenP, sh, f, m, n, lenF, order, cap, PMLG_METHOD_AUTO), fn);
+    free(f);
+    for (slong i = 0; i < m; i++)
+        shift[i] = (slong)sh[i];
+    free(sh);
+    unpack(appbas, P, cap);
+    free(P);
+}
+
+void nmod_poly_mat_pmbasis(nmod_poly_mat_t appbas, slong *shift, const nmod_poly_mat_t F,
+                           slong order)
+{
+    basis_impl(appbas, shift, F, order, 0, "nmod_poly_mat_pmbasis");
+}
+
+void nmod_poly_mat_mbasis(nmod_poly_mat_t appbas, slong *shift, const nmod_poly_mat_t F,
+                          ulong order)
+{
+    basis_impl(appbas, shift, F, (slong)order, 1, "nmod_poly_mat_mbasis");
+}
+
+/* ------------------------------------------------------------------------ */
+/* certificate (verification.c:21-116)                                      */
+/* ------------------------------------------------------------------------ */
+
+static uint64_t mulmod_h(uint64_t a, uint64_t b, uint64_t p)
+{
+    return (uint64_t)(((unsigned __int128)a * b) % p);
+}
+
+static uint64_t powmod_h(uint64_t a, uint64_t e, uint64_t p)
+{
+    uint64_t r = 1 % p;
+    for (a %= p; e; e >>= 1, a = mulmod_h(a, a, p))
+        if (e & 1)
+            r = mulmod_h(r, a, p);
+    return r;
+}
+
+static uint64_t splitmix64(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+static slong entry_degree(const nmod_poly_struct *e)
+{
+    slong l = e->length;
+    while (l > 0 && e->coeffs[l - 1] == 0)
+        l--;
+    return l - 1;
+}
+
+/* Returns 1 when appbas is a shift-ordered weak Popov approximant basis of F
+ * at `order` (ROW_LOWER), else 0, printing which test failed, as the
+ * reference does.  The residual appbas*F mod x^(order+1) is a device middle
+ * product; det(appbas) = lambda x^D (D the sum of the diagonal degrees) is
+ * checked at x = 1 and three seeded points by device evaluations and device
+ * determinants; the rank of [appbas(0) | coefficient `order` of appbas*F] is a
+ * device elimination. */
+int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_poly_mat_t F,
+                                       slong order, const slong *shift, orientation_t orient)
+{
+    static const char *fn = "nmod_poly_mat_is_approximant_basis";
+    const slong m = F->r, n = F->c;
+    const uint64_t p = F->modulus;
+    int success = 1;
+    if (orient != ROW_LOWER)
+        die(fn, "only the ROW_LOWER orientation is supported");
+    if (appbas->r != m || appbas->c != m) {
+        printf("basis has wrong row dimension or column dimension\n");
+        return 0;
+    }
+
+    /* shifted ordered weak Popov: pivot of row i (rightmost entry reaching the
+     * shifted row degree) strictly increasing */
+    slong prev = -1;
+    for (slong i = 0; i < m; i++) {
+        slong piv = -1, mx = 0;
+        for (slong j = 0; j < m; j++) {
+            const slong d = entry_degree(nmod_poly_mat_entry(appbas, i, j));
+            if (d >= 0 && (piv == -1 || mx <= d + shift[j])) {
+                piv = j;
+                mx = d + shift[j];
+            }
+        }
+        if (piv == -1 || piv <= prev) {
+            printf("basis is not shifted-weak Popov\n");
+            success = 0;
+            break;
+        }
+        prev = piv;
+    }
+    if (m == 0)
+        return success;
+
+    pmlg_ctx ctx = ctx_for(p, fn);
+    slong lenP = pmlg_nmod_poly_mat_max_length(appbas);
+    slong lenF = pmlg_nmod_poly_mat_max_length(F);
+    const slong lenR = order + 1;
+    uint64_t *R = xalloc((size_t)(m * n * lenR), fn);
+    if (lenP == 0 || lenF == 0 || n == 0) {
+        memset(R, 0, (size_t)(m * n * lenR) * sizeof(uint64_t));
+    } else {
+        lenF = lenF < lenR ? lenF : lenR;
+        const slong lp = lenP < lenR ? lenP : lenR;
+        uint64_t *a = pack(appbas, lp, fn);
+        uint64_t *f = pack(F, lenF, fn);
+        check(pmlg_polmat_mulmid(ctx, R, a, f, m, m, n, lp, lenF, 0, lenR, PMLG_METHOD_AUTO), fn);
+        free(a);
+        free(f);
+    }
+    for (slong e = 0; e < m * n && success >= 0; e++)
+        for (slong d = 0; d < order; d++)
+            if (R[e * lenR + d]) {
+                printf("entry %ld, %ld of residual has valuation less than the order\n",
+                       (long)(e / (n ? n : 1)), (long)(e % (n ? n : 1)));
+                success = 0;
+                e = m * n;
+                break;
+            }
+
+    /* det(appbas) = lambda x^D */
+    slong D = 0;
+    for (slong i = 0; i < m; i++)
+        D += entry_degree(nmod_poly_mat_entry(appbas, i, i));
+    {
+        uint64_t *P = pack(appbas, lenP, fn);
+        uint64_t *E = xalloc((size_t)(m * m), fn);
+        int64_t rk = 0;
+        uint64_t lambda = 0, dv = 0;
+        check(pmlg_polmat_eval(ctx, E, P, m, m, lenP, 1 % p), fn);
+        check(pmlg_const_rank_det(ctx, E, m, m, &rk, &lambda), fn);
+        int ok = lambda != 0 && D >= 0;
+        uint64_t st = 0x5eed5eed5eed5eedULL;
+        for (int t = 0; t < 3 && ok; t++) {
+            const uint64_t x = p > 2 ? 2 + splitmix64(&st) % (p - 2) : 1 % p;
+            check(pmlg_polmat_eval(ctx, E, P, m, m, lenP, x), fn);
+            check(pmlg_const_rank_det(ctx, E, m, m, &rk, &dv), fn);
+            ok = dv == mulmod_h(lambda, powmod_h(x, (uint64_t)D, p), p);
+        }
+        if (!ok) {
+            printf("determinant is not lambda * x**(sum(diag-deg))\n");
+            success = 0;
+        }
+        free(E);
+        free(P);
+    }
+
+    /* [appbas(0) | C] of full rank, C = coefficient `order` of the residual */
+    {
+        uint64_t *CP = xalloc((size_t)(m * (m + n)), fn);
+        for (slong i = 0; i < m; i++) {
+            for (slong j = 0; j < m; j++) {
+                const nmod_poly_struct *e = nmod_poly_mat_entry(appbas, i, j);
+                CP[i * (m + n) + j] = e->length > 0 ? e->coeffs[0] : 0;
+            }
+            for (slong j = 0; j < n; j++)
+                CP[i * (m + n) + m + j] = R[(i * n + j) * lenR + order];
+        }
+        int64_t rk = 0;
+        check(pmlg_const_rank_det(ctx, CP, m, m + n, &rk, NULL), fn);
+        if (rk != m) {
+            printf("generation test (step: [C P(0)] has full rank) failed\n");
+            success = 0;
+        }
+        free(CP);
+    }
+    free(R);
+    return success;
+}

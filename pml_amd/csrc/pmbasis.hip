@@ -654,3 +654,152 @@ pmlg_status pmlg_mbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Constant matrices and evaluation (used by the approximant-basis certificate)
+// ===========================================================================
+namespace pmlg {
+namespace {
+
+// Gaussian elimination of an r x c matrix held in device memory, by one
+// workgroup (the matrix is modified in place; fcol holds r words of scratch):
+// rank, and the determinant when r == c (product of the pivots, sign of the
+// row swaps).  Barriers order the global-memory updates of one workgroup.
+__global__ void __launch_bounds__(1024) const_gauss_kernel(uint64_t *A, uint64_t *fcol, int r, int c,
+                                                           int *out_rank, uint64_t *out_det,
+                                                           ModConst mc)
+{
+    __shared__ int s_pr;
+    __shared__ uint64_t s_inv;
+    const int tid = threadIdx.x;
+    const uint64_t p = mc.p;
+    int rank = 0;
+    uint64_t det = 1 % p;
+    bool neg = false;
+    for (int col = 0; col < c && rank < r; ++col) {
+        if (tid == 0) {
+            int pr = -1;
+            for (int i = rank; i < r; ++i)
+                if (A[(size_t)i * c + col]) {
+                    pr = i;
+                    break;
+                }
+            s_pr = pr;
+            if (pr >= 0)
+                s_inv = invmod_g(A[(size_t)pr * c + col], mc);
+        }
+        __syncthreads();
+        const int pr = s_pr;
+        if (pr < 0) {
+            det = 0;
+            __syncthreads();
+            continue;
+        }
+        if (pr != rank) {
+            for (int j = tid; j < c; j += blockDim.x) {
+                const uint64_t t = A[(size_t)pr * c + j];
+                A[(size_t)pr * c + j] = A[(size_t)rank * c + j];
+                A[(size_t)rank * c + j] = t;
+            }
+            neg = !neg;
+        }
+        __syncthreads();
+        det = mulmod_g(det, A[(size_t)rank * c + col], mc);
+        for (int i = tid; i < r; i += blockDim.x)
+            fcol[i] = mulmod_g(A[(size_t)i * c + col], s_inv, mc);
+        __syncthreads();
+        for (int64_t idx = (int64_t)(rank + 1) * c + tid; idx < (int64_t)r * c; idx += blockDim.x) {
+            const int i = (int)(idx / c), j = (int)(idx - (int64_t)i * c);
+            A[idx] = submod_g(A[idx], mulmod_g(fcol[i], A[(size_t)rank * c + j], mc), p);
+        }
+        ++rank;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        *out_rank = rank;
+        if (out_det)
+            *out_det = (r == c && rank == r) ? (neg && det ? p - det : det) : 0;
+    }
+}
+
+// out[e] = sum_d A[e*len + d] x^d (Horner), one thread per entry
+__global__ void eval_kernel(uint64_t *out, const uint64_t *A, int64_t entries, int64_t len,
+                            uint64_t x, ModConst mc)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < entries;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t v = 0;
+        for (int64_t d = len - 1; d >= 0; --d) {
+            v = mulmod_g(v, x, mc) + A[e * len + d];
+            v = v >= mc.p ? v - mc.p : v;
+        }
+        out[e] = v;
+    }
+}
+
+}  // namespace
+}  // namespace pmlg
+
+extern "C" {
+
+pmlg_status pmlg_const_rank_det(pmlg_ctx ctx, const uint64_t *M, int64_t r, int64_t c,
+                                int64_t *rank, uint64_t *det)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (r < 0 || c < 0 || r > (1 << 16) || c > (1 << 16) || r * c > (1 << 22) || !rank ||
+        (r * c > 0 && !M))
+        return fail(PMLG_ERR_ARG, "const_rank_det: at most 2^22 entries");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t words = (size_t)(r * c) + (size_t)r + 4;
+    HIP_TRY(ctx->in_b.ensure(words * 8), "hipMalloc");
+    uint64_t *dM = static_cast<uint64_t *>(ctx->in_b.ptr);
+    uint64_t *dcol = dM + r * c;
+    uint64_t *ddet = dcol + r;
+    int *drank = reinterpret_cast<int *>(ddet + 1);
+    if (r * c)
+        HIP_TRY(hipMemcpyAsync(dM, M, (size_t)(r * c) * 8, hipMemcpyHostToDevice, s), "H2D");
+    hipLaunchKernelGGL(const_gauss_kernel, dim3(1), dim3(1024), 0, s, dM, dcol, (int)r, (int)c,
+                       drank, ddet, ctx->mc);
+    HIP_TRY(hipGetLastError(), "const_gauss_kernel");
+    int hr = 0;
+    uint64_t hd = 0;
+    HIP_TRY(hipMemcpyAsync(&hr, drank, sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+    HIP_TRY(hipMemcpyAsync(&hd, ddet, 8, hipMemcpyDeviceToHost, s), "D2H");
+    HIP_TRY(hipStreamSynchronize(s), "sync");
+    *rank = hr;
+    if (det)
+        *det = hd;
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_polmat_eval(pmlg_ctx ctx, uint64_t *out, const uint64_t *A, int64_t r, int64_t c,
+                             int64_t len, uint64_t x)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (r < 0 || c < 0 || len < 0 || !out || (r * c * len > 0 && !A))
+        return fail(PMLG_ERR_ARG, "polmat_eval: bad arguments");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t ab = (size_t)(r * c * len) * 8, ob = (size_t)(r * c) * 8;
+    HIP_TRY(ctx->in_a.ensure(ab ? ab : 8), "hipMalloc");
+    HIP_TRY(ctx->out_c.ensure(ob ? ob : 8), "hipMalloc");
+    if (ab)
+        HIP_TRY(hipMemcpyAsync(ctx->in_a.ptr, A, ab, hipMemcpyHostToDevice, s), "H2D");
+    hipLaunchKernelGGL(eval_kernel, dim3(grid_for(r * c)), dim3(256), 0, s,
+                       static_cast<uint64_t *>(ctx->out_c.ptr),
+                       static_cast<const uint64_t *>(ctx->in_a.ptr), r * c, len, x % ctx->p, ctx->mc);
+    HIP_TRY(hipGetLastError(), "eval_kernel");
+    if (ob)
+        HIP_TRY(hipMemcpyAsync(out, ctx->out_c.ptr, ob, hipMemcpyDeviceToHost, s), "D2H");
+    HIP_TRY(hipStreamSynchronize(s), "sync");
+    return PMLG_OK;
+}
+
+}  // extern "C"

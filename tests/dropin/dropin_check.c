@@ -1,0 +1,278 @@
+/*
+ * Parity check of the FLINT-shaped drop-in (include/pmlgpu_flint.h) against
+ * the CPU oracle, written the way a PML user calls the library: FLINT
+ * polynomial matrices in, FLINT polynomial matrices out.
+ *
+ * Follows the reference's own tests of these entry points
+ * (flint-extras/src/nmod_poly_mat_extra/test/t-mul_geometric.c,
+ * t-mul_waksman.c, t-mulmid.c, t-pmbasis.c, t-mbasis.c: random matrices,
+ * equality with a second method, is_approximant_basis on the output) with the
+ * oracle as the second method.
+ *
+ * Prints "ok <case>" per passing case; exits 1 at the first mismatch.
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pml_oracle.h"
+#include "pmlgpu_flint.h"
+
+static const uint64_t P60 = 1152921504556515329ULL;
+static const uint64_t P30 = 998244353ULL;
+static uint64_t g_state = 12345;
+
+static void fail(const char *what)
+{
+    printf("FAIL %s\n", what);
+    fflush(stdout);
+    exit(1);
+}
+
+/* random matrix; entry lengths uniform in [0, maxlen] (ragged) or all maxlen */
+static void rand_mat(nmod_poly_mat_t M, slong maxlen, int ragged, int zero_frac)
+{
+    for (slong i = 0; i < M->r; i++)
+        for (slong j = 0; j < M->c; j++) {
+            nmod_poly_struct *e = nmod_poly_mat_entry(M, i, j);
+            slong len = ragged ? (slong)(orc_splitmix64(&g_state) % (uint64_t)(maxlen + 1)) : maxlen;
+            if (zero_frac && orc_splitmix64(&g_state) % 4 == 0)
+                len = 0;
+            e->length = 0;
+            if (len == 0)
+                continue;
+            nmod_poly_fit_length(e, len);
+            orc_rand_uniform(e->coeffs, len, M->modulus, orc_splitmix64(&g_state));
+            if (e->coeffs[len - 1] == 0)
+                e->coeffs[len - 1] = 1;
+            e->length = len;
+        }
+}
+
+static uint64_t *pack(const nmod_poly_mat_t M, slong len)
+{
+    uint64_t *b = calloc((size_t)(M->r * M->c * len + 1), sizeof(uint64_t));
+    for (slong i = 0; i < M->r; i++)
+        for (slong j = 0; j < M->c; j++) {
+            const nmod_poly_struct *e = nmod_poly_mat_entry(M, i, j);
+            slong l = e->length < len ? e->length : len;
+            if (l > 0)
+                memcpy(b + (i * M->c + j) * len, e->coeffs, (size_t)l * sizeof(uint64_t));
+        }
+    return b;
+}
+
+/* entries of M equal the oracle's buffer (normalised lengths) */
+static void expect_equal(const nmod_poly_mat_t M, const uint64_t *buf, slong len, const char *what)
+{
+    for (slong i = 0; i < M->r; i++)
+        for (slong j = 0; j < M->c; j++) {
+            const uint64_t *s = buf + (i * M->c + j) * len;
+            slong l = len;
+            while (l > 0 && s[l - 1] == 0)
+                l--;
+            const nmod_poly_struct *e = nmod_poly_mat_entry(M, i, j);
+            if (e->length != l) {
+                printf("entry (%ld,%ld): length %ld, oracle %ld\n", (long)i, (long)j,
+                       (long)e->length, (long)l);
+                fail(what);
+            }
+            if (l && memcmp(e->coeffs, s, (size_t)l * sizeof(uint64_t))) {
+                printf("entry (%ld,%ld) differs\n", (long)i, (long)j);
+                fail(what);
+            }
+        }
+}
+
+typedef void (*mul_fn)(nmod_poly_mat_t, const nmod_poly_mat_t, const nmod_poly_mat_t);
+
+static void check_mul(mul_fn f, const char *name, slong m, slong k, slong n, slong len, int ragged,
+                      int zeros, uint64_t p)
+{
+    nmod_poly_mat_t A, B, C;
+    nmod_poly_mat_init(A, m, k, p);
+    nmod_poly_mat_init(B, k, n, p);
+    nmod_poly_mat_init(C, m, n, p);
+    rand_mat(A, len, ragged, zeros);
+    rand_mat(B, len, ragged, zeros);
+    /* stale content in the output must be overwritten */
+    rand_mat(C, 7, 1, 0);
+    f(C, A, B);
+    const slong la = nmod_poly_mat_max_length(A), lb = nmod_poly_mat_max_length(B);
+    const slong lc = (la && lb) ? la + lb - 1 : 1;
+    uint64_t *a = pack(A, la ? la : 1), *b = pack(B, lb ? lb : 1);
+    uint64_t *c = calloc((size_t)(m * n * lc + 1), sizeof(uint64_t));
+    if (la && lb)
+        orc_polmat_mul(c, a, b, m, k, n, la, lb, p);
+    expect_equal(C, c, lc, name);
+    printf("ok %s %ldx%ldx%ld len<=%ld ragged=%d zeros=%d p=%llu\n", name, (long)m, (long)k,
+           (long)n, (long)len, ragged, zeros, (unsigned long long)p);
+    free(a);
+    free(b);
+    free(c);
+    nmod_poly_mat_clear(A);
+    nmod_poly_mat_clear(B);
+    nmod_poly_mat_clear(C);
+}
+
+static void check_mul_alias(slong d, slong len, uint64_t p)
+{
+    nmod_poly_mat_t A, B;
+    nmod_poly_mat_init(A, d, d, p);
+    nmod_poly_mat_init(B, d, d, p);
+    rand_mat(A, len, 1, 0);
+    rand_mat(B, len, 1, 0);
+    const slong la = nmod_poly_mat_max_length(A), lb = nmod_poly_mat_max_length(B);
+    uint64_t *a = pack(A, la), *b = pack(B, lb);
+    uint64_t *c = calloc((size_t)(d * d * (la + lb - 1)), sizeof(uint64_t));
+    orc_polmat_mul(c, a, b, d, d, d, la, lb, p);
+    nmod_poly_mat_multiply(A, A, B); /* res aliases the left operand */
+    expect_equal(A, c, la + lb - 1, "multiply aliasing res == A");
+    uint64_t *a2 = pack(A, la + lb - 1);
+    uint64_t *c2 = calloc((size_t)(d * d * (2 * la + 2 * lb - 3)), sizeof(uint64_t));
+    orc_polmat_mul(c2, a2, a2, d, d, d, la + lb - 1, la + lb - 1, p);
+    nmod_poly_mat_multiply(A, A, A); /* both operands alias the output */
+    expect_equal(A, c2, 2 * la + 2 * lb - 3, "multiply aliasing res == A == B");
+    printf("ok multiply aliasing %ldx%ld len<=%ld\n", (long)d, (long)d, (long)len);
+    free(a);
+    free(b);
+    free(c);
+    free(a2);
+    free(c2);
+    nmod_poly_mat_clear(A);
+    nmod_poly_mat_clear(B);
+}
+
+static void check_mulmid(slong m, slong k, slong n, slong lenA, slong lenB, slong nlo, slong nhi,
+                         int underscore, uint64_t p)
+{
+    nmod_poly_mat_t A, B, R;
+    nmod_poly_mat_init(A, m, k, p);
+    nmod_poly_mat_init(B, k, n, p);
+    nmod_poly_mat_init(R, m, n, p);
+    rand_mat(A, lenA, 1, 1);
+    rand_mat(B, lenB, 1, 1);
+    rand_mat(R, 5, 1, 0);
+    const slong la = nmod_poly_mat_max_length(A), lb = nmod_poly_mat_max_length(B);
+    if (underscore)
+        _nmod_poly_mat_mulmid(R, A, la, B, lb, nlo, nhi);
+    else
+        nmod_poly_mat_mulmid(R, A, B, nlo, nhi);
+    const slong lr = nhi > nlo ? nhi - nlo : 1;
+    uint64_t *a = pack(A, la ? la : 1), *b = pack(B, lb ? lb : 1);
+    uint64_t *r = calloc((size_t)(m * n * lr + 1), sizeof(uint64_t));
+    if (la && lb && nhi > nlo)
+        orc_polmat_mulmid(r, a, b, m, k, n, la, lb, nlo, nhi, p);
+    expect_equal(R, r, lr, underscore ? "_nmod_poly_mat_mulmid" : "nmod_poly_mat_mulmid");
+    printf("ok %smulmid %ldx%ldx%ld len %ld,%ld [%ld,%ld) p=%llu\n", underscore ? "_" : "",
+           (long)m, (long)k, (long)n, (long)lenA, (long)lenB, (long)nlo, (long)nhi,
+           (unsigned long long)p);
+    free(a);
+    free(b);
+    free(r);
+    nmod_poly_mat_clear(A);
+    nmod_poly_mat_clear(B);
+    nmod_poly_mat_clear(R);
+}
+
+static void check_basis(int iterative, slong m, slong n, slong lenF, slong order, slong shift_step,
+                        uint64_t p)
+{
+    const char *name = iterative ? "nmod_poly_mat_mbasis" : "nmod_poly_mat_pmbasis";
+    nmod_poly_mat_t F, P;
+    nmod_poly_mat_init(F, m, n, p);
+    nmod_poly_mat_init(P, m, m, p);
+    rand_mat(F, lenF, 0, 0);
+    slong *shift = malloc((size_t)m * sizeof(slong)), *shift0 = malloc((size_t)m * sizeof(slong));
+    int64_t *oshift = malloc((size_t)m * sizeof(int64_t));
+    for (slong i = 0; i < m; i++)
+        shift[i] = shift0[i] = oshift[i] = (shift_step * i) % 7;
+    if (iterative)
+        nmod_poly_mat_mbasis(P, shift, F, (ulong)order);
+    else
+        nmod_poly_mat_pmbasis(P, shift, F, order);
+    uint64_t *f = pack(F, lenF);
+    uint64_t *OP = NULL;
+    int64_t olen = 0;
+    if (iterative)
+        orc_mbasis(&OP, &olen, oshift, f, m, n, lenF, order, p);
+    else
+        orc_pmbasis(&OP, &olen, oshift, f, m, n, lenF, order, p);
+    expect_equal(P, OP, olen, name);
+    for (slong i = 0; i < m; i++)
+        if (shift[i] != oshift[i])
+            fail("shifted row degree differs from the oracle");
+    if (!nmod_poly_mat_is_approximant_basis(P, F, order, shift0, ROW_LOWER))
+        fail("is_approximant_basis rejects the computed basis");
+    /* the oracle's certificate agrees */
+    uint64_t *pp = pack(P, olen);
+    if (orc_is_approximant_basis(pp, olen, f, m, n, lenF, order, shift0, p, 7) != 0)
+        fail("oracle certificate rejects the computed basis");
+    printf("ok %s %ldx%ld order %ld p=%llu\n", name, (long)m, (long)n, (long)order,
+           (unsigned long long)p);
+
+    /* x * (row 0): still approximants, still weak Popov with the same pivots
+     * and det = lambda x^(D+1) with D+1 the new diagonal degree sum, but
+     * [P(0) | C] loses rank: only the generation test catches it */
+    nmod_poly_struct *e;
+    for (slong j = 0; j < m; j++) {
+        e = nmod_poly_mat_entry(P, 0, j);
+        if (e->length == 0)
+            continue;
+        nmod_poly_fit_length(e, e->length + 1);
+        memmove(e->coeffs + 1, e->coeffs, (size_t)e->length * sizeof(uint64_t));
+        e->coeffs[0] = 0;
+        e->length++;
+    }
+    if (nmod_poly_mat_is_approximant_basis(P, F, order, shift0, ROW_LOWER))
+        fail("is_approximant_basis accepts x * row 0");
+    /* a changed low coefficient breaks the residual */
+    e = nmod_poly_mat_entry(P, m - 1, m - 1);
+    if (e->length == 0) {
+        nmod_poly_fit_length(e, 1);
+        e->coeffs[0] = 0;
+        e->length = 1;
+    }
+    e->coeffs[0] = (e->coeffs[0] + 1) % p;
+    while (e->length > 0 && e->coeffs[e->length - 1] == 0)
+        e->length--;
+    if (nmod_poly_mat_is_approximant_basis(P, F, order, shift0, ROW_LOWER))
+        fail("is_approximant_basis accepts a perturbed basis");
+    printf("ok is_approximant_basis rejects perturbed bases %ldx%ld\n", (long)m, (long)n);
+    orc_free(OP);
+    free(f);
+    free(pp);
+    free(shift);
+    free(shift0);
+    free(oshift);
+    nmod_poly_mat_clear(F);
+    nmod_poly_mat_clear(P);
+}
+
+int main(void)
+{
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 5, 7, 3, 300, 1, 1, P60);
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 16, 16, 16, 1000, 0, 0, P60);
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 9, 4, 11, 513, 1, 0, P30);
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 1, 6, 1, 64, 1, 1, P30);
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 3, 3, 3, 1, 0, 0, P60);
+    check_mul(nmod_poly_mat_mul_geometric, "nmod_poly_mat_mul_geometric", 6, 5, 4, 200, 1, 0, P60);
+    check_mul(nmod_poly_mat_mul_vandermonde1, "nmod_poly_mat_mul_vandermonde1", 6, 5, 4, 200, 1, 0, P60);
+    check_mul(nmod_poly_mat_mul_vandermonde2, "nmod_poly_mat_mul_vandermonde2", 6, 5, 4, 200, 1, 0, P30);
+    check_mul(nmod_poly_mat_mul_waksman, "nmod_poly_mat_mul_waksman", 6, 5, 4, 200, 1, 0, P30);
+    check_mul_alias(6, 100, P60);
+    check_mulmid(7, 5, 6, 200, 300, 100, 300, 0, P60);
+    check_mulmid(7, 5, 6, 200, 300, 0, 1000, 0, P60); /* nhi beyond the product length */
+    check_mulmid(4, 4, 4, 900, 50, 10, 60, 0, P30);   /* left operand longer than nhi */
+    check_mulmid(4, 4, 4, 50, 60, 80, 80, 0, P30);    /* empty range */
+    check_mulmid(8, 8, 4, 65, 128, 64, 128, 1, P30);  /* PM-Basis residual shape */
+    check_mulmid(3, 2, 5, 40, 400, 39, 400, 1, P60);
+    check_basis(0, 6, 3, 100, 100, 3, P60);
+    check_basis(0, 8, 4, 150, 150, 0, P30);
+    check_basis(0, 5, 5, 64, 64, 1, P30); /* square F */
+    check_basis(1, 4, 2, 40, 40, 2, P30);
+    check_basis(1, 7, 3, 25, 25, 0, P60);
+    printf("all drop-in checks passed\n");
+    return 0;
+}
