@@ -49,7 +49,8 @@ typedef enum {
 typedef enum {
     PMLG_METHOD_AUTO = 0,  /* MFMA when the shape and modulus allow it, plain otherwise */
     PMLG_METHOD_PLAIN = 1, /* per-lane exact 128-bit (or 64-bit) dot products */
-    PMLG_METHOD_MFMA = 2   /* int8-limb MFMA contraction with int32 accumulation */
+    PMLG_METHOD_MFMA = 2,  /* int8-limb MFMA contraction with int32 accumulation */
+    PMLG_METHOD_THRESHOLD = 3 /* batched products: the hand-set threshold rule only */
 } pmlg_method;
 
 typedef struct pmlg_ctx_s *pmlg_ctx;
@@ -117,6 +118,59 @@ pmlg_status pmlg_const_rank_det(pmlg_ctx ctx, const uint64_t *M, int64_t r, int6
 /* out[i][j] = A[i][j](x) for an r x c polynomial matrix (host pointers). */
 pmlg_status pmlg_polmat_eval(pmlg_ctx ctx, uint64_t *out, const uint64_t *A, int64_t r, int64_t c,
                              int64_t len, uint64_t x);
+
+/* ------------------------------------------------------------------------
+ * Batched products (no reference counterpart: the reference multiplies one
+ * pair per nmod_poly_mat_multiply call, mul.c:19-97).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    pmlg_ctx ctx;       /* modulus of this product; every context of a batch on one device */
+    const uint64_t *A;  /* device, m x k entries of padded length lenA */
+    const uint64_t *B;  /* device, k x n entries of padded length lenB */
+    uint64_t *C;        /* device, m x n entries of padded length lenA + lenB - 1 */
+    int64_t m, k, n, lenA, lenB;
+    int method;         /* PMLG_METHOD_*: AUTO = learned model, autotuned cache, then threshold */
+    int config;         /* >= 0: force configuration `config` (0 .. pmlg_num_configs()-1) */
+} pmlg_mul_desc;
+
+typedef struct pmlg_batch_plan_s *pmlg_batch_plan;
+
+#define PMLG_BATCH_GRAPH 1 /* capture the launches into a hipGraph on the first run */
+
+/* Plans `count` independent products C_i = A_i * B_i (nothing is written for
+ * a product with lenA or lenB 0).  The plan allocates the evaluation arena
+ * and keeps the descriptors' device pointers: they must stay valid while the
+ * plan is run.  Products are grouped so that one launch per stage serves all
+ * products sharing a modulus and transform length. */
+pmlg_status pmlg_batch_plan_create(pmlg_batch_plan *plan, const pmlg_mul_desc *descs,
+                                   int64_t count, int flags);
+/* Enqueues the whole batch on `stream` (NULL: the plan's stream); with
+ * PMLG_BATCH_GRAPH it replays one captured graph.  Does not synchronise. */
+pmlg_status pmlg_batch_plan_run(pmlg_batch_plan plan, void *stream);
+/* Waits for the plan's own stream. */
+pmlg_status pmlg_batch_plan_sync(pmlg_batch_plan plan);
+/* Configuration chosen for product i (-1 for an empty product). */
+pmlg_status pmlg_batch_plan_config(pmlg_batch_plan plan, int64_t i, int *config);
+/* Kernel launches per run and arena bytes. */
+pmlg_status pmlg_batch_plan_stats(pmlg_batch_plan plan, int64_t *launches, uint64_t *arena_bytes);
+void pmlg_batch_plan_destroy(pmlg_batch_plan plan);
+/* One-shot: plan, run, synchronise, destroy. */
+pmlg_status pmlg_polmat_mul_batch(const pmlg_mul_desc *descs, int64_t count, void *stream);
+
+/* Configurations: 0 direct convolution; 1..7 transforms with the plain or
+ * MFMA pointwise kernel and different transform radices / MFMA block sizes. */
+int pmlg_num_configs(void);
+int pmlg_config_valid(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                      int config);
+/* Configuration the dispatcher picks for this shape under `method`. */
+int pmlg_choose_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                       int method);
+/* Autotuner: times every valid configuration on device-generated inputs of
+ * this shape (one warm-up and `reps` timed runs each; ms_per_config[c] = -1
+ * for a configuration not timed), records the fastest for the shape class in
+ * the context, and returns it in *best_config. */
+pmlg_status pmlg_autotune(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA,
+                          int64_t lenB, int reps, double *ms_per_config, int *best_config);
 
 /* Per-stage device timing (HIP events around each launch).  Stages:
  * 0 forward NTT, 1 pointwise product, 2 inverse NTT (+ merge). */

@@ -23,8 +23,18 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmlgpu.so")
 
-METHOD_AUTO, METHOD_PLAIN, METHOD_MFMA = 0, 1, 2
-_METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "mfma": METHOD_MFMA}
+METHOD_AUTO, METHOD_PLAIN, METHOD_MFMA, METHOD_THRESHOLD = 0, 1, 2, 3
+_METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "mfma": METHOD_MFMA,
+            "threshold": METHOD_THRESHOLD}
+BATCH_GRAPH = 1
+
+
+class MulDesc(ctypes.Structure):
+    """pmlg_mul_desc: one product of a batch (device pointers)."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("A", ctypes.c_void_p), ("B", ctypes.c_void_p),
+                ("C", ctypes.c_void_p), ("m", ctypes.c_int64), ("k", ctypes.c_int64),
+                ("n", ctypes.c_int64), ("lenA", ctypes.c_int64), ("lenB", ctypes.c_int64),
+                ("method", ctypes.c_int), ("config", ctypes.c_int)]
 
 STATUS = {0: "PMLG_OK", 1: "PMLG_ERR_ARG", 2: "PMLG_ERR_UNSUPPORTED", 3: "PMLG_ERR_DEVICE",
           4: "PMLG_ERR_ALLOC"}
@@ -85,6 +95,30 @@ def lib():
         L.pmlg_memcpy_h2d.argtypes = [vp, vp, vp, u64]
         L.pmlg_memcpy_d2h.restype = st
         L.pmlg_memcpy_d2h.argtypes = [vp, vp, vp, u64]
+        L.pmlg_batch_plan_create.restype = st
+        L.pmlg_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(MulDesc), i64,
+                                             c_int]
+        L.pmlg_batch_plan_run.restype = st
+        L.pmlg_batch_plan_run.argtypes = [vp, vp]
+        L.pmlg_batch_plan_sync.restype = st
+        L.pmlg_batch_plan_sync.argtypes = [vp]
+        L.pmlg_batch_plan_config.restype = st
+        L.pmlg_batch_plan_config.argtypes = [vp, i64, ctypes.POINTER(c_int)]
+        L.pmlg_batch_plan_stats.restype = st
+        L.pmlg_batch_plan_stats.argtypes = [vp, _i64p, _u64p]
+        L.pmlg_batch_plan_destroy.restype = None
+        L.pmlg_batch_plan_destroy.argtypes = [vp]
+        L.pmlg_polmat_mul_batch.restype = st
+        L.pmlg_polmat_mul_batch.argtypes = [ctypes.POINTER(MulDesc), i64, vp]
+        L.pmlg_num_configs.restype = c_int
+        L.pmlg_num_configs.argtypes = []
+        L.pmlg_config_valid.restype = c_int
+        L.pmlg_config_valid.argtypes = [vp, i64, i64, i64, i64, i64, c_int]
+        L.pmlg_choose_config.restype = c_int
+        L.pmlg_choose_config.argtypes = [vp, i64, i64, i64, i64, i64, c_int]
+        L.pmlg_autotune.restype = st
+        L.pmlg_autotune.argtypes = [vp, i64, i64, i64, i64, i64, c_int,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int)]
         if hasattr(L, "pmlg_pmbasis"):
             L.pmlg_pmbasis.restype = st
             L.pmlg_pmbasis.argtypes = [vp, _u64p, _i64p, _i64p, _u64p, i64, i64, i64, i64, i64,
@@ -248,3 +282,123 @@ class Context:
         cnt = (ctypes.c_int64 * 3)()
         _check(lib().pmlg_profile_read(self.handle, ms, cnt))
         return list(ms), list(cnt)
+
+
+# ---------------------------------------------------------------------------
+# batched products
+# ---------------------------------------------------------------------------
+def num_configs():
+    return lib().pmlg_num_configs()
+
+
+class BatchPlan:
+    """A planned batch of independent products (pmlg_batch_plan).
+
+    `items` is a list of (context, A, B) with A, B numpy arrays of shape
+    (m, k, lenA) and (k, n, lenB); inputs are uploaded once and stay resident,
+    outputs are read back by results().  `method` / `config` apply to every
+    product unless a per-item sequence is given.
+    """
+
+    def __init__(self, items, method="auto", config=-1, graph=True):
+        self.items = []
+        self._bufs = []
+        descs = (MulDesc * max(len(items), 1))()
+        methods = method if isinstance(method, (list, tuple)) else [method] * len(items)
+        configs = config if isinstance(config, (list, tuple)) else [config] * len(items)
+        for i, (ctx, A, B) in enumerate(items):
+            A, B = _c(A), _c(B)
+            m, k, la = A.shape
+            k2, n, lb = B.shape
+            if k != k2:
+                raise ValueError("inner dimensions differ in item %d" % i)
+            lc = la + lb - 1 if la > 0 and lb > 0 else 0
+            dA = DeviceArray(ctx, max(A.nbytes, 8))
+            dB = DeviceArray(ctx, max(B.nbytes, 8))
+            dC = DeviceArray(ctx, max(m * n * lc * 8, 8))
+            if A.nbytes:
+                dA.upload(A)
+            if B.nbytes:
+                dB.upload(B)
+            self._bufs += [dA, dB, dC]
+            self.items.append((ctx, (m, k, n, la, lb, lc), dC))
+            d = descs[i]
+            d.ctx, d.A, d.B, d.C = ctx.handle, dA.ptr, dB.ptr, dC.ptr
+            d.m, d.k, d.n, d.lenA, d.lenB = m, k, n, la, lb
+            d.method = _method(methods[i])
+            d.config = int(configs[i])
+        h = ctypes.c_void_p()
+        _check(lib().pmlg_batch_plan_create(ctypes.byref(h), descs, len(items),
+                                            BATCH_GRAPH if graph else 0))
+        self.handle = h.value
+
+    def run(self, stream=None):
+        _check(lib().pmlg_batch_plan_run(self.handle, stream))
+
+    def sync(self):
+        _check(lib().pmlg_batch_plan_sync(self.handle))
+
+    def config(self, i):
+        c = ctypes.c_int()
+        _check(lib().pmlg_batch_plan_config(self.handle, i, ctypes.byref(c)))
+        return c.value
+
+    def stats(self):
+        l, b = ctypes.c_int64(), ctypes.c_uint64()
+        _check(lib().pmlg_batch_plan_stats(self.handle, ctypes.byref(l), ctypes.byref(b)))
+        return l.value, b.value
+
+    def result(self, i):
+        ctx, (m, k, n, la, lb, lc), dC = self.items[i]
+        C = np.zeros((m, n, lc), dtype=np.uint64)
+        if C.nbytes:
+            dC.download(C)
+        return C
+
+    def results(self):
+        return [self.result(i) for i in range(len(self.items))]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().pmlg_batch_plan_destroy(self.handle)
+            self.handle = None
+        for b in self._bufs:
+            b.free()
+        self._bufs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def mul_batch(items, method="auto", config=-1, graph=False):
+    """Products of a batch through one plan run; returns the list of C_i."""
+    with BatchPlan(items, method=method, config=config, graph=graph) as plan:
+        plan.run(None)
+        plan.sync()
+        return plan.results()
+
+
+def autotune(ctx, m, k, n, lenA, lenB, reps=2):
+    """Times every valid configuration; returns (ms list with -1 for untimed, best)."""
+    K = num_configs()
+    ms = (ctypes.c_double * K)()
+    best = ctypes.c_int()
+    _check(lib().pmlg_autotune(ctx.handle, m, k, n, lenA, lenB, reps, ms, ctypes.byref(best)))
+    return list(ms), best.value
+
+
+def choose_config(ctx, m, k, n, lenA, lenB, method="auto"):
+    return lib().pmlg_choose_config(ctx.handle, m, k, n, lenA, lenB, _method(method))
+
+
+def config_valid(ctx, m, k, n, lenA, lenB, config):
+    return bool(lib().pmlg_config_valid(ctx.handle, m, k, n, lenA, lenB, config))

@@ -348,7 +348,7 @@ pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
                               lenB, nlo, nhi, method, s);
 }
 
-}  // namespace
+}  // namespace pmlg
 
 // ---------------------------------------------------------------------------
 // extern "C" API
@@ -421,6 +421,8 @@ void pmlg_ctx_destroy(pmlg_ctx ctx)
         ctx->ev_c.release();
         ctx->tmp.release();
         ctx->arena.release();
+        ctx->tune_io.release();
+        ctx->tune_ws.release();
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -1,6 +1,7 @@
 // Context object shared by the host translation units of libpmlgpu.
 #pragma once
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -97,6 +98,10 @@ struct pmlg_ctx_s {
     std::vector<pmlg::Prof> pending;
     double ms[3] = {0, 0, 0};
     int64_t launches[3] = {0, 0, 0};
+    // autotuner: best configuration per shape class, and its timing buffers
+    std::mutex tune_mu;
+    std::map<uint64_t, int> tuned;
+    pmlg::DevBuf tune_io, tune_ws;
 };
 
 namespace pmlg {

@@ -1,0 +1,39 @@
+// Per-product configurations of the batched path and the dispatch between
+// them (threshold rule, autotuner cache, learned cost model).
+#pragma once
+
+#include "ctx.hpp"
+
+namespace pmlg {
+
+struct ProdConfig {
+    int school;          // 1: direct convolution, 0: transforms + pointwise products
+    PointwiseKind kind;  // pointwise kernel
+    int fwd_e, inv_e;    // log2 elements per thread of the register transforms
+    int pb_shift;        // MFMA points per workgroup: the largest that fits, >> pb_shift
+};
+
+constexpr int kNumConfigs = 8;
+extern const ProdConfig kConfigs[kNumConfigs];
+
+// direct convolution is offered when the shorter operand has at most this
+// many coefficients, chosen by the threshold rule up to kSchoolThreshold, and
+// timed by the autotuner while m n k lenC min(lenA, lenB) <= kSchoolTimeCap
+constexpr int64_t kSchoolMaxLen = 256;
+constexpr int64_t kSchoolThreshold = 8;
+constexpr double kSchoolTimeCap = 8.6e9;
+
+bool config_valid(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB, int c);
+int threshold_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB);
+int choose_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                  int method);
+// configuration picked by the learned cost model, or -1 when none is loaded
+int learned_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB);
+
+pmlg_status build_plan(pmlg_batch_plan *out, const pmlg_mul_desc *descs, int64_t count, int flags,
+                       void *ws, size_t ws_bytes, size_t *need_bytes);
+pmlg_status run_plan(pmlg_batch_plan plan, hipStream_t s);
+pmlg_status time_configs(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                         int reps, double *ms, int *best);
+
+}  // namespace pmlg

@@ -57,6 +57,49 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
                           int64_t count, int logN, int logP, const NttTune &tune,
                           const NttTab<W> &tab, W p, W *tmp, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Segment tables of the batched launches.  One launch serves many operand
+// matrices (segments) that share a prime and a transform length; segment s
+// owns the workgroups [block0, block0 + blocks) of the launch, found by a
+// binary search on block0 (segments sorted by block0, no gaps).
+// ---------------------------------------------------------------------------
+
+// Forward: `count` entries, entry e at in + e*in_stride (first in_len words
+// used, zero beyond), evaluations in blocked layout at out.
+template <typename Win, typename W>
+struct FwdSeg {
+    const Win *in;
+    W *out;
+    int64_t in_stride, in_len, count, block0;
+};
+
+// Inverse: `count` entries of blocked evaluations at in; coefficients [lo, hi)
+// of entry e to out + e*out_stride.  tmp (count*N words) when N exceeds the
+// LDS limit; the merge pass owns workgroups [mblock0, ...) of its own launch.
+template <typename W, typename Wout>
+struct InvSeg {
+    const W *in;
+    Wout *out;
+    W *tmp;
+    int64_t out_stride, lo, hi, count, block0, mblock0;
+};
+
+// Workgroups one segment of `count` entries needs at length 2^logN (transform
+// launch), and for the merge pass (0 when N fits in LDS).
+template <typename W>
+int64_t ntt_seg_blocks(int64_t count, int logN);
+template <typename W>
+int64_t merge_seg_blocks(int64_t count, int64_t width, int logN);
+
+// segs: device array of nseg segments; blocks / mblocks: totals of the above.
+template <typename Win, typename W>
+hipError_t launch_ntt_fwd_segs(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logN,
+                               int logP, int log_e, const NttTab<W> &tab, W p, hipStream_t s);
+template <typename W, typename Wout>
+hipError_t launch_ntt_inv_segs(const InvSeg<W, Wout> *segs, int nseg, int64_t blocks,
+                               int64_t mblocks, int logN, int logP, int log_e,
+                               const NttTab<W> &tab, W p, hipStream_t s);
+
 enum PointwiseKind { PW_PLAIN = 0, PW_MFMA = 1 };
 
 // C_hat[i][j][q] = sum_k A_hat[i][k][q] * B_hat[k][j][q] mod p, q < N.
@@ -66,5 +109,28 @@ hipError_t launch_pointwise(PointwiseKind kind, const W *Ah, const W *Bh, W *Ch,
 
 // Whether the int8-limb MFMA kernel supports this shape.
 bool mfma_supported(int m, int k, int n, int limbs, uint64_t p);
+
+// One product of a batched pointwise launch (evaluations in blocked layout,
+// N points); tasks [block0, block0 + pw_seg_blocks(...)) are its own.
+template <typename W>
+struct PwSeg {
+    const W *Ah, *Bh;
+    W *Ch;
+    int32_t m, k, n, logP;
+    int64_t N, block0;
+    int32_t redc_small, pad;
+};
+int64_t pw_seg_blocks(PointwiseKind kind, int pb, int m, int n, int64_t N);
+// LDS bytes of an MFMA task with pb points per workgroup
+size_t pw_mfma_lds(int pb, int m, int k, int n, int limbs);
+// largest points-per-workgroup (<= 2^logP) whose limb planes fit 64 KiB
+int pw_pick_pb(int m, int k, int n, int limbs, int logP);
+// Persistent launch over all tasks of nseg products sharing one prime:
+// `grid` workgroups (0: one per task) walk the task list; MFMA tasks use pb
+// points per workgroup and `lds` bytes (the maximum over the products).
+template <typename W>
+hipError_t launch_pointwise_segs(PointwiseKind kind, int pb, const PwSeg<W> *segs, int nseg,
+                                 int64_t blocks, size_t lds, int grid, const ModConst &mc,
+                                 hipStream_t s);
 
 }  // namespace pmlg

@@ -41,11 +41,13 @@ __device__ __forceinline__ W tw_mul(W x, const W *w, const W *wq, int64_t idx, W
     return shoup_lazy(x, w[idx], wq[idx], p);
 }
 
+// Units u0 .. u0+nb-1 (of `units` = count << logR) of one operand: the
+// workgroup body shared by the direct and the segment-table launches.
 template <typename Win, typename W>
-__global__ void __launch_bounds__(kThreads)
-ntt_fwd_kernel(const Win *__restrict__ in, int64_t in_stride, int64_t in_len,
-               W *__restrict__ out, int64_t units, int logN, int logR, int lognb, int logP,
-               NttTab<W> tab, W p)
+__device__ __forceinline__ void fwd_lds_body(const Win *__restrict__ in, int64_t in_stride,
+                                             int64_t in_len, W *__restrict__ out, int64_t units,
+                                             int64_t u0, int logN, int logR, int lognb, int logP,
+                                             const NttTab<W> &tab, W p)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *s = reinterpret_cast<W *>(smem_raw);
@@ -55,7 +57,6 @@ ntt_fwd_kernel(const Win *__restrict__ in, int64_t in_stride, int64_t in_len,
     const int nb = 1 << lognb;
     const int64_t N = (int64_t)1 << logN;
     const W p2 = p + p;
-    const int64_t u0 = (int64_t)blockIdx.x * nb;
 
     // load (with the radix-R split when R > 1)
     for (int idx = threadIdx.x; idx < nb * M; idx += kThreads) {
@@ -116,11 +117,22 @@ ntt_fwd_kernel(const Win *__restrict__ in, int64_t in_stride, int64_t in_len,
     }
 }
 
-template <typename W, typename Wout>
+template <typename Win, typename W>
 __global__ void __launch_bounds__(kThreads)
-ntt_inv_kernel(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
-               int64_t hi, W *__restrict__ tmp, int64_t units, int logN, int logR, int lognb,
-               int logP, NttTab<W> tab, W p)
+ntt_fwd_kernel(const Win *__restrict__ in, int64_t in_stride, int64_t in_len,
+               W *__restrict__ out, int64_t units, int logN, int logR, int lognb, int logP,
+               NttTab<W> tab, W p)
+{
+    fwd_lds_body<Win, W>(in, in_stride, in_len, out, units, (int64_t)blockIdx.x << lognb, logN,
+                         logR, lognb, logP, tab, p);
+}
+
+template <typename W, typename Wout>
+__device__ __forceinline__ void inv_lds_body(const W *__restrict__ in, Wout *__restrict__ out,
+                                             int64_t out_stride, int64_t lo, int64_t hi,
+                                             W *__restrict__ tmp, int64_t units, int64_t u0,
+                                             int logN, int logR, int lognb, int logP,
+                                             const NttTab<W> &tab, W p)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *s = reinterpret_cast<W *>(smem_raw);
@@ -130,7 +142,6 @@ ntt_inv_kernel(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_str
     const int nb = 1 << lognb;
     const int64_t N = (int64_t)1 << logN;
     const W p2 = p + p;
-    const int64_t u0 = (int64_t)blockIdx.x * nb;
 
     for (int idx = threadIdx.x; idx < nb * M; idx += kThreads) {
         const int t = idx >> logM;
@@ -184,32 +195,49 @@ ntt_inv_kernel(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_str
     }
 }
 
+template <typename W, typename Wout>
+__global__ void __launch_bounds__(kThreads)
+ntt_inv_kernel(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
+               int64_t hi, W *__restrict__ tmp, int64_t units, int logN, int logR, int lognb,
+               int logP, NttTab<W> tab, W p)
+{
+    inv_lds_body<W, Wout>(in, out, out_stride, lo, hi, tmp, units, (int64_t)blockIdx.x << lognb,
+                          logN, logR, lognb, logP, tab, p);
+}
+
 // x[o] = N^-1 * sum_g w^(-(o g) mod N) z_g[o mod M], o in [lo, hi)
+template <typename W, typename Wout>
+__device__ __forceinline__ void merge_one(const W *__restrict__ z, Wout *__restrict__ out,
+                                          int64_t out_stride, int64_t lo, int64_t width,
+                                          int64_t idx, int logN, int logR, const NttTab<W> &tab,
+                                          W p)
+{
+    const int64_t N = (int64_t)1 << logN;
+    const int64_t M = (int64_t)1 << (logN - logR);
+    const int R = 1 << logR;
+    const W p2 = p + p;
+    const int64_t e = idx / width;
+    const int64_t o = lo + (idx - e * width);
+    const int64_t i1 = o & (M - 1);
+    const W *ze = z + e * N;
+    W acc = reduce_2p<W>(ze[i1], p);
+    for (int g = 1; g < R; ++g) {
+        const int64_t widx = (o * g) & (N - 1);
+        acc = add_2p<W>(acc, tw_mul<W>(ze[(int64_t)g * M + i1], tab.itw, tab.itwq, widx, p), p2);
+    }
+    out[e * out_stride + (o - lo)] = (Wout)shoup<W>(acc, tab.ninv, tab.ninvq, p);
+}
+
 template <typename W, typename Wout>
 __global__ void __launch_bounds__(kThreads)
 ntt_merge_kernel(const W *__restrict__ z, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
                  int64_t hi, int64_t count, int logN, int logR, NttTab<W> tab, W p)
 {
-    const int64_t N = (int64_t)1 << logN;
-    const int logM = logN - logR;
-    const int64_t M = (int64_t)1 << logM;
-    const int R = 1 << logR;
     const int64_t width = hi - lo;
     const int64_t total = count * width;
-    const W p2 = p + p;
     for (int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * kThreads) {
-        const int64_t e = idx / width;
-        const int64_t o = lo + (idx - e * width);
-        const int64_t i1 = o & (M - 1);
-        const W *ze = z + e * N;
-        W acc = reduce_2p<W>(ze[i1], p);
-        for (int g = 1; g < R; ++g) {
-            const int64_t widx = (o * g) & (N - 1);
-            acc = add_2p<W>(acc, tw_mul<W>(ze[(int64_t)g * M + i1], tab.itw, tab.itwq, widx, p), p2);
-        }
-        out[e * out_stride + (o - lo)] = (Wout)shoup<W>(acc, tab.ninv, tab.ninvq, p);
-    }
+         idx += (int64_t)gridDim.x * kThreads)
+        merge_one<W, Wout>(z, out, out_stride, lo, width, idx, logN, logR, tab, p);
 }
 
 inline int pick_lognb(int logM)
@@ -462,15 +490,15 @@ __device__ __forceinline__ void inv_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 <<
 }
 
 template <typename Win, typename W, bool LOOSE, int L, int LOGE>
-__global__ void __launch_bounds__(1 << (L - LOGE))
-ntt_fwd_reg(const Win *__restrict__ in, int64_t in_stride, int64_t in_len, W *__restrict__ out,
-            int64_t units, int logR, int logP, NttTab<W> tab, W p)
+__device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t in_stride,
+                                             int64_t in_len, W *__restrict__ out, int64_t units,
+                                             int64_t u, int logR, int logP, const NttTab<W> &tab,
+                                             W p)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
     constexpr int E = 1 << LOGE, T = 1 << (L - LOGE), M = 1 << L;
     const int t = threadIdx.x;
-    const int64_t u = xcd_unit(blockIdx.x, units);
     const int64_t e = u >> logR;
     const int R = 1 << logR;
     const int gg = (int)(u & (R - 1));
@@ -514,16 +542,25 @@ ntt_fwd_reg(const Win *__restrict__ in, int64_t in_stride, int64_t in_len, W *__
     }
 }
 
-template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
+template <typename Win, typename W, bool LOOSE, int L, int LOGE>
 __global__ void __launch_bounds__(1 << (L - LOGE))
-ntt_inv_reg(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
-            int64_t hi, W *__restrict__ tmp, int64_t units, int logR, int logP, NttTab<W> tab, W p)
+ntt_fwd_reg(const Win *__restrict__ in, int64_t in_stride, int64_t in_len, W *__restrict__ out,
+            int64_t units, int logR, int logP, NttTab<W> tab, W p)
+{
+    fwd_reg_body<Win, W, LOOSE, L, LOGE>(in, in_stride, in_len, out, units,
+                                         xcd_unit(blockIdx.x, units), logR, logP, tab, p);
+}
+
+template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
+__device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__restrict__ out,
+                                             int64_t out_stride, int64_t lo, int64_t hi,
+                                             W *__restrict__ tmp, int64_t units, int64_t u,
+                                             int logR, int logP, const NttTab<W> &tab, W p)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
     constexpr int E = 1 << LOGE, T = 1 << (L - LOGE), M = 1 << L;
     const int t = threadIdx.x;
-    const int64_t u = xcd_unit(blockIdx.x, units);
     const int64_t e = u >> logR;
     const int R = 1 << logR;
     const int gg = (int)(u & (R - 1));
@@ -557,6 +594,15 @@ ntt_inv_reg(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride
         for (int j = 0; j < E; ++j)
             o[(j << (L - LOGE)) | t] = bfly_final<W, LOOSE>(x[j], k);
     }
+}
+
+template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
+__global__ void __launch_bounds__(1 << (L - LOGE))
+ntt_inv_reg(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
+            int64_t hi, W *__restrict__ tmp, int64_t units, int logR, int logP, NttTab<W> tab, W p)
+{
+    inv_reg_body<W, Wout, LOOSE, L, LOGE>(in, out, out_stride, lo, hi, tmp, units,
+                                          xcd_unit(blockIdx.x, units), logR, logP, tab, p);
 }
 
 // Register-kernel launchers.  logE (elements per thread) is a tuning
@@ -767,6 +813,304 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
                        tmp, out, out_stride, lo, hi, count, logN, logR, tab, p);
     return hipGetLastError();
 }
+
+// ===========================================================================
+// Segment-table launches (batched products): same workgroup bodies, the
+// operand matrix of each workgroup looked up by a binary search on block0.
+// ===========================================================================
+namespace {
+
+template <typename S>
+__device__ __forceinline__ const S &seg_of(const S *segs, int nseg, int64_t b)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].block0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return segs[lo];
+}
+
+template <typename S>
+__device__ __forceinline__ const S &mseg_of(const S *segs, int nseg, int64_t b)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].mblock0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return segs[lo];
+}
+
+template <typename Win, typename W>
+__global__ void __launch_bounds__(kThreads)
+ntt_fwd_segs_lds(const FwdSeg<Win, W> *__restrict__ segs, int nseg, int logN, int logR, int lognb,
+                 int logP, NttTab<W> tab, W p)
+{
+    const FwdSeg<Win, W> &sg = seg_of(segs, nseg, blockIdx.x);
+    fwd_lds_body<Win, W>(sg.in, sg.in_stride, sg.in_len, sg.out, sg.count << logR,
+                         ((int64_t)blockIdx.x - sg.block0) << lognb, logN, logR, lognb, logP, tab,
+                         p);
+}
+
+template <typename W, typename Wout>
+__global__ void __launch_bounds__(kThreads)
+ntt_inv_segs_lds(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logN, int logR,
+                 int lognb, int logP, NttTab<W> tab, W p)
+{
+    const InvSeg<W, Wout> &sg = seg_of(segs, nseg, blockIdx.x);
+    inv_lds_body<W, Wout>(sg.in, sg.out, sg.out_stride, sg.lo, sg.hi, sg.tmp, sg.count << logR,
+                          ((int64_t)blockIdx.x - sg.block0) << lognb, logN, logR, lognb, logP,
+                          tab, p);
+}
+
+template <typename Win, typename W, bool LOOSE, int L, int LOGE>
+__global__ void __launch_bounds__(1 << (L - LOGE))
+ntt_fwd_segs_reg(const FwdSeg<Win, W> *__restrict__ segs, int nseg, int logR, int logP,
+                 NttTab<W> tab, W p)
+{
+    const FwdSeg<Win, W> &sg = seg_of(segs, nseg, blockIdx.x);
+    fwd_reg_body<Win, W, LOOSE, L, LOGE>(sg.in, sg.in_stride, sg.in_len, sg.out, sg.count << logR,
+                                         (int64_t)blockIdx.x - sg.block0, logR, logP, tab, p);
+}
+
+template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
+__global__ void __launch_bounds__(1 << (L - LOGE))
+ntt_inv_segs_reg(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logR, int logP,
+                 NttTab<W> tab, W p)
+{
+    const InvSeg<W, Wout> &sg = seg_of(segs, nseg, blockIdx.x);
+    inv_reg_body<W, Wout, LOOSE, L, LOGE>(sg.in, sg.out, sg.out_stride, sg.lo, sg.hi, sg.tmp,
+                                          sg.count << logR, (int64_t)blockIdx.x - sg.block0, logR,
+                                          logP, tab, p);
+}
+
+template <typename W, typename Wout>
+__global__ void __launch_bounds__(kThreads)
+ntt_merge_segs(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logN, int logR,
+               NttTab<W> tab, W p)
+{
+    const InvSeg<W, Wout> &sg = mseg_of(segs, nseg, blockIdx.x);
+    const int64_t width = sg.hi - sg.lo;
+    const int64_t idx = ((int64_t)blockIdx.x - sg.mblock0) * kThreads + threadIdx.x;
+    if (idx < sg.count * width)
+        merge_one<W, Wout>(sg.tmp, sg.out, sg.out_stride, sg.lo, width, idx, logN, logR, tab, p);
+}
+
+template <typename Kern>
+hipError_t set_lds_attr(Kern *k, size_t lds, bool &done)
+{
+    if (done || lds <= 64 * 1024)
+        return hipSuccess;
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e == hipSuccess)
+        done = true;
+    return e;
+}
+
+template <typename Win, typename W, bool LOOSE, int L, int LOGE>
+hipError_t fwd_segs_reg3(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR, int logP,
+                         const NttTab<W> &tab, W p, hipStream_t s)
+{
+    const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
+    static bool attr = false;
+    hipError_t e = set_lds_attr(ntt_fwd_segs_reg<Win, W, LOOSE, L, LOGE>, lds, attr);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL((ntt_fwd_segs_reg<Win, W, LOOSE, L, LOGE>), dim3((unsigned)blocks),
+                       dim3(1 << (L - LOGE)), lds, s, segs, nseg, logR, logP, tab, p);
+    return hipGetLastError();
+}
+
+template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
+hipError_t inv_segs_reg3(const InvSeg<W, Wout> *segs, int nseg, int64_t blocks, int logR, int logP,
+                         const NttTab<W> &tab, W p, hipStream_t s)
+{
+    const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
+    static bool attr = false;
+    hipError_t e = set_lds_attr(ntt_inv_segs_reg<W, Wout, LOOSE, L, LOGE>, lds, attr);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL((ntt_inv_segs_reg<W, Wout, LOOSE, L, LOGE>), dim3((unsigned)blocks),
+                       dim3(1 << (L - LOGE)), lds, s, segs, nseg, logR, logP, tab, p);
+    return hipGetLastError();
+}
+
+template <typename Win, typename W, bool LOOSE, int L>
+hipError_t fwd_segs_reg2(int logE, const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR,
+                         int logP, const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if (logE == 5 && L >= 12)
+        return fwd_segs_reg3<Win, W, LOOSE, L, (L >= 12 ? 5 : 4)>(segs, nseg, blocks, logR, logP,
+                                                                 tab, p, s);
+    return fwd_segs_reg3<Win, W, LOOSE, L, 4>(segs, nseg, blocks, logR, logP, tab, p, s);
+}
+
+template <typename W, typename Wout, bool LOOSE, int L>
+hipError_t inv_segs_reg2(int logE, const InvSeg<W, Wout> *segs, int nseg, int64_t blocks,
+                         int logR, int logP, const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if (logE == 5 && L >= 12)
+        return inv_segs_reg3<W, Wout, LOOSE, L, (L >= 12 ? 5 : 4)>(segs, nseg, blocks, logR, logP,
+                                                                  tab, p, s);
+    return inv_segs_reg3<W, Wout, LOOSE, L, 4>(segs, nseg, blocks, logR, logP, tab, p, s);
+}
+
+template <typename Win, typename W, int L>
+hipError_t fwd_segs_reg(int logE, const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR,
+                        int logP, const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if constexpr (sizeof(W) == 8) {
+        if ((uint64_t)p < ((uint64_t)1 << 61))
+            return fwd_segs_reg2<Win, W, true, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
+    }
+    return fwd_segs_reg2<Win, W, false, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
+}
+
+template <typename W, typename Wout, int L>
+hipError_t inv_segs_reg(int logE, const InvSeg<W, Wout> *segs, int nseg, int64_t blocks, int logR,
+                        int logP, const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if constexpr (sizeof(W) == 8) {
+        if ((uint64_t)p < ((uint64_t)1 << 61))
+            return inv_segs_reg2<W, Wout, true, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
+    }
+    return inv_segs_reg2<W, Wout, false, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
+}
+
+template <typename W>
+int seg_log_m(int logN)
+{
+    return logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
+}
+
+}  // namespace
+
+template <typename W>
+int64_t ntt_seg_blocks(int64_t count, int logN)
+{
+    const int logM = seg_log_m<W>(logN);
+    const int64_t units = count << (logN - logM);
+    if (logM >= kRegMinLog)
+        return units;
+    const int lognb = pick_lognb(logM);
+    return (units + (1 << lognb) - 1) >> lognb;
+}
+
+template <typename W>
+int64_t merge_seg_blocks(int64_t count, int64_t width, int logN)
+{
+    if (logN <= NttLimits<W>::max_log_m || width <= 0)
+        return 0;
+    return (count * width + kThreads - 1) / kThreads;
+}
+
+template <typename Win, typename W>
+hipError_t launch_ntt_fwd_segs(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logN,
+                               int logP, int log_e, const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if (nseg <= 0 || blocks <= 0)
+        return hipSuccess;
+    const int logM = seg_log_m<W>(logN);
+    const int logR = logN - logM;
+    if (logR > 2 || logP > logN || blocks > INT32_MAX)
+        return hipErrorInvalidValue;
+    if (logM >= kRegMinLog) {
+        switch (logM) {
+        case 11:
+            return fwd_segs_reg<Win, W, 11>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+        case 12:
+            return fwd_segs_reg<Win, W, 12>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+        case 13:
+            return fwd_segs_reg<Win, W, 13>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+        case 14:
+            if constexpr (sizeof(W) == 4)
+                return fwd_segs_reg<Win, W, 14>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            [[fallthrough]];
+        default:
+            return hipErrorInvalidValue;
+        }
+    }
+    const int lognb = pick_lognb(logM);
+    const size_t lds = ((size_t)1 << (logM + lognb)) * sizeof(W);
+    hipLaunchKernelGGL((ntt_fwd_segs_lds<Win, W>), dim3((unsigned)blocks), dim3(kThreads), lds, s,
+                       segs, nseg, logN, logR, lognb, logP, tab, p);
+    return hipGetLastError();
+}
+
+template <typename W, typename Wout>
+hipError_t launch_ntt_inv_segs(const InvSeg<W, Wout> *segs, int nseg, int64_t blocks,
+                               int64_t mblocks, int logN, int logP, int log_e,
+                               const NttTab<W> &tab, W p, hipStream_t s)
+{
+    if (nseg <= 0 || blocks <= 0)
+        return hipSuccess;
+    const int logM = seg_log_m<W>(logN);
+    const int logR = logN - logM;
+    if (logR > 2 || logP > logN || blocks > INT32_MAX || mblocks > INT32_MAX)
+        return hipErrorInvalidValue;
+    hipError_t err;
+    if (logM >= kRegMinLog) {
+        switch (logM) {
+        case 11:
+            err = inv_segs_reg<W, Wout, 11>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            break;
+        case 12:
+            err = inv_segs_reg<W, Wout, 12>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            break;
+        case 13:
+            err = inv_segs_reg<W, Wout, 13>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            break;
+        case 14:
+            if constexpr (sizeof(W) == 4) {
+                err = inv_segs_reg<W, Wout, 14>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+                break;
+            }
+            [[fallthrough]];
+        default:
+            return hipErrorInvalidValue;
+        }
+    } else {
+        const int lognb = pick_lognb(logM);
+        const size_t lds = ((size_t)1 << (logM + lognb)) * sizeof(W);
+        hipLaunchKernelGGL((ntt_inv_segs_lds<W, Wout>), dim3((unsigned)blocks), dim3(kThreads), lds,
+                           s, segs, nseg, logN, logR, lognb, logP, tab, p);
+        err = hipGetLastError();
+    }
+    if (err != hipSuccess || logR == 0 || mblocks <= 0)
+        return err;
+    hipLaunchKernelGGL((ntt_merge_segs<W, Wout>), dim3((unsigned)mblocks), dim3(kThreads), 0, s,
+                       segs, nseg, logN, logR, tab, p);
+    return hipGetLastError();
+}
+
+template int64_t ntt_seg_blocks<uint64_t>(int64_t, int);
+template int64_t ntt_seg_blocks<uint32_t>(int64_t, int);
+template int64_t merge_seg_blocks<uint64_t>(int64_t, int64_t, int);
+template int64_t merge_seg_blocks<uint32_t>(int64_t, int64_t, int);
+template hipError_t launch_ntt_fwd_segs<uint64_t, uint64_t>(const FwdSeg<uint64_t, uint64_t> *, int,
+                                                            int64_t, int, int, int,
+                                                            const NttTab<uint64_t> &, uint64_t,
+                                                            hipStream_t);
+template hipError_t launch_ntt_fwd_segs<uint64_t, uint32_t>(const FwdSeg<uint64_t, uint32_t> *, int,
+                                                            int64_t, int, int, int,
+                                                            const NttTab<uint32_t> &, uint32_t,
+                                                            hipStream_t);
+template hipError_t launch_ntt_inv_segs<uint64_t, uint64_t>(const InvSeg<uint64_t, uint64_t> *, int,
+                                                            int64_t, int64_t, int, int, int,
+                                                            const NttTab<uint64_t> &, uint64_t,
+                                                            hipStream_t);
+template hipError_t launch_ntt_inv_segs<uint32_t, uint64_t>(const InvSeg<uint32_t, uint64_t> *, int,
+                                                            int64_t, int64_t, int, int, int,
+                                                            const NttTab<uint32_t> &, uint32_t,
+                                                            hipStream_t);
 
 template hipError_t launch_ntt_fwd<uint64_t, uint64_t>(const uint64_t *, int64_t, int64_t,
                                                        uint64_t *, int64_t, int, int,

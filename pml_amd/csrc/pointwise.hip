@@ -44,13 +44,14 @@ __device__ __forceinline__ void plain_fold(uint64_t &hi, uint64_t &lo, const Mod
     hi = 0;
 }
 
-__global__ void __launch_bounds__(kThreads)
-pw_plain_u64(const uint64_t *__restrict__ Ah, const uint64_t *__restrict__ Bh,
-             uint64_t *__restrict__ Ch, int m, int k, int n, int64_t N, int logP, ModConst mc)
+// Lane (point q, output tile `tile` of TM x TN entries) of the plain product.
+__device__ __forceinline__ void plain_u64_body(const uint64_t *__restrict__ Ah,
+                                               const uint64_t *__restrict__ Bh,
+                                               uint64_t *__restrict__ Ch, int m, int k, int n,
+                                               int64_t N, int logP, const ModConst &mc, int64_t q,
+                                               int tile)
 {
-    const int64_t q = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     const int tiles_n = (n + TN - 1) / TN;
-    const int tile = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int i0 = (tile / tiles_n) * TM;
     const int j0 = (tile % tiles_n) * TN;
     if (q >= N || i0 >= m)
@@ -93,13 +94,13 @@ pw_plain_u64(const uint64_t *__restrict__ Ah, const uint64_t *__restrict__ Bh,
                     reduce128(hi[r][c], lo[r][c], mc);
 }
 
-__global__ void __launch_bounds__(kThreads)
-pw_plain_u32(const uint32_t *__restrict__ Ah, const uint32_t *__restrict__ Bh,
-             uint32_t *__restrict__ Ch, int m, int k, int n, int64_t N, int logP, ModConst mc)
+__device__ __forceinline__ void plain_u32_body(const uint32_t *__restrict__ Ah,
+                                               const uint32_t *__restrict__ Bh,
+                                               uint32_t *__restrict__ Ch, int m, int k, int n,
+                                               int64_t N, int logP, const ModConst &mc, int64_t q,
+                                               int tile)
 {
-    const int64_t q = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     const int tiles_n = (n + TN - 1) / TN;
-    const int tile = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int i0 = (tile / tiles_n) * TM;
     const int j0 = (tile % tiles_n) * TN;
     if (q >= N || i0 >= m)
@@ -140,6 +141,20 @@ pw_plain_u32(const uint32_t *__restrict__ Ah, const uint32_t *__restrict__ Bh,
             if (i0 + r < m && j0 + c < n)
                 Ch[blk_addr((int64_t)(i0 + r) * n + j0 + c, q, (int64_t)m * n, logP)] =
                     (uint32_t)barrett(acc[r][c], mc.bm, mc.p);
+}
+
+// grid: x = 64-point chunks, y = groups of 4 output tiles (one per wave)
+template <typename W>
+__global__ void __launch_bounds__(kThreads)
+pw_plain(const W *__restrict__ Ah, const W *__restrict__ Bh, W *__restrict__ Ch, int m, int k,
+         int n, int64_t N, int logP, ModConst mc)
+{
+    const int64_t q = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int tile = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if constexpr (sizeof(W) == 8)
+        plain_u64_body(Ah, Bh, Ch, m, k, n, N, logP, mc, q, tile);
+    else
+        plain_u32_body(Ah, Bh, Ch, m, k, n, N, logP, mc, q, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -257,22 +272,18 @@ __device__ __forceinline__ int mfma_row(int r, int lane)
     return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 }
 
+// Point block `blk` (points blk*PB .. blk*PB+PB-1) of the MFMA product, one
+// workgroup; the caller separates two calls on one workgroup by a barrier.
 template <typename W, int L, int PB>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
-pw_mfma_kernel(const W *__restrict__ Ah, const W *__restrict__ Bh, W *__restrict__ Ch, int m,
-               int k, int n, int64_t N, int logP, ModConst mc)
+__device__ __forceinline__ void mfma_body(const W *__restrict__ Ah, const W *__restrict__ Bh,
+                                          W *__restrict__ Ch, int m, int k, int n, int logP,
+                                          const ModConst &mc, int64_t blk)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     const int Mp = (m + 31) & ~31, Kp = (k + 31) & ~31, Np = (n + 31) & ~31;
     const int Kq = Kp >> 2;
     unsigned char *la = lds;                             // [PB][L][Mp][Kp]
     unsigned char *lb = lds + (size_t)PB * L * Mp * Kp;  // [PB][L][Np][Kp]  (B transposed)
-    // XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (round-robin
-    // dispatch), so give them consecutive point blocks to share its L2
-    const int64_t nblk = N / PB;
-    int64_t blk = blockIdx.x;
-    if ((nblk & 7) == 0)
-        blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
     const int64_t q0 = blk * PB;
     const int tid = threadIdx.x;
 
@@ -413,6 +424,71 @@ pw_mfma_kernel(const W *__restrict__ Ah, const W *__restrict__ Bh, W *__restrict
     }
 }
 
+template <typename W, int L, int PB>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+pw_mfma_kernel(const W *__restrict__ Ah, const W *__restrict__ Bh, W *__restrict__ Ch, int m,
+               int k, int n, int64_t N, int logP, ModConst mc)
+{
+    // XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (round-robin
+    // dispatch), so give them consecutive point blocks to share its L2
+    const int64_t nblk = N / PB;
+    int64_t blk = blockIdx.x;
+    if ((nblk & 7) == 0)
+        blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
+    mfma_body<W, L, PB>(Ah, Bh, Ch, m, k, n, logP, mc, blk);
+}
+
+// ---------------------------------------------------------------------------
+// persistent segment-table launches (batched products)
+// ---------------------------------------------------------------------------
+template <typename W>
+__device__ __forceinline__ const PwSeg<W> &pw_seg_of(const PwSeg<W> *segs, int nseg, int64_t b)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].block0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return segs[lo];
+}
+
+// Every workgroup walks the task list b = blockIdx.x, + gridDim.x, ... until
+// `blocks`; a task is one point block of one product.
+template <typename W, int L, int PB>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+pw_mfma_segs(const PwSeg<W> *__restrict__ segs, int nseg, int64_t blocks, ModConst mc)
+{
+    for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+        const PwSeg<W> &sg = pw_seg_of(segs, nseg, b);
+        ModConst mcs = mc;
+        mcs.redc_small = sg.redc_small;
+        mfma_body<W, L, PB>(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.logP, mcs, b - sg.block0);
+        __syncthreads();  // the next task rewrites the limb planes
+    }
+}
+
+// task = (64-point chunk, group of 4 output tiles) of one product
+template <typename W>
+__global__ void __launch_bounds__(kThreads)
+pw_plain_segs(const PwSeg<W> *__restrict__ segs, int nseg, int64_t blocks, ModConst mc)
+{
+    for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+        const PwSeg<W> &sg = pw_seg_of(segs, nseg, b);
+        const int64_t gx = (sg.N + 63) >> 6;
+        const int64_t lb = b - sg.block0;
+        const int64_t by = lb / gx;
+        const int64_t q = (lb - by * gx) * 64 + (threadIdx.x & 63);
+        const int tile = (int)by * 4 + (threadIdx.x >> 6);
+        if constexpr (sizeof(W) == 8)
+            plain_u64_body(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.N, sg.logP, mc, q, tile);
+        else
+            plain_u32_body(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.N, sg.logP, mc, q, tile);
+    }
+}
+
 inline int64_t mfma_point_bytes(int m, int k, int n, int limbs)
 {
     const int64_t Mp = (m + 31) & ~31, Kp = (k + 31) & ~31, Np = (n + 31) & ~31;
@@ -485,8 +561,8 @@ hipError_t launch_pointwise<uint64_t>(PointwiseKind kind, const uint64_t *Ah, co
     }
     const int tiles = ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
     dim3 grid((unsigned)((N + 63) / 64), (unsigned)((tiles + 3) / 4));
-    hipLaunchKernelGGL(pw_plain_u64, grid, dim3(kThreads), 0, s, Ah, Bh, Ch, m, k, n, N, logP,
-                       mc);
+    hipLaunchKernelGGL(pw_plain<uint64_t>, grid, dim3(kThreads), 0, s, Ah, Bh, Ch, m, k, n, N,
+                       logP, mc);
     return hipGetLastError();
 }
 
@@ -502,9 +578,93 @@ hipError_t launch_pointwise<uint32_t>(PointwiseKind kind, const uint32_t *Ah, co
     }
     const int tiles = ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
     dim3 grid((unsigned)((N + 63) / 64), (unsigned)((tiles + 3) / 4));
-    hipLaunchKernelGGL(pw_plain_u32, grid, dim3(kThreads), 0, s, Ah, Bh, Ch, m, k, n, N, logP,
-                       mc);
+    hipLaunchKernelGGL(pw_plain<uint32_t>, grid, dim3(kThreads), 0, s, Ah, Bh, Ch, m, k, n, N,
+                       logP, mc);
     return hipGetLastError();
 }
+
+
+// ---------------------------------------------------------------------------
+// batched launches
+// ---------------------------------------------------------------------------
+int64_t pw_seg_blocks(PointwiseKind kind, int pb, int m, int n, int64_t N)
+{
+    if (kind == PW_MFMA)
+        return N / pb;
+    const int tiles = ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
+    return ((N + 63) / 64) * ((tiles + 3) / 4);
+}
+
+size_t pw_mfma_lds(int pb, int m, int k, int n, int limbs)
+{
+    return (size_t)pb * mfma_point_bytes(m, k, n, limbs);
+}
+
+int pw_pick_pb(int m, int k, int n, int limbs, int logP)
+{
+    const int64_t per = mfma_point_bytes(m, k, n, limbs);
+    constexpr int64_t pref = 64 * 1024;
+    for (int pb = 8; pb > 1; pb >>= 1)
+        if (pb <= (1 << logP) && pb * per <= pref)
+            return pb;
+    return 1;
+}
+
+namespace {
+template <typename W, int L, int PB>
+hipError_t mfma_segs_pb(const PwSeg<W> *segs, int nseg, int64_t blocks, size_t lds, int grid,
+                        const ModConst &mc, hipStream_t s)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)pw_mfma_segs<W, L, PB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBudget);
+        if (e != hipSuccess)
+            return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((pw_mfma_segs<W, L, PB>), dim3((unsigned)grid), dim3(kThreads), lds, s, segs,
+                       nseg, blocks, mc);
+    return hipGetLastError();
+}
+}  // namespace
+
+template <typename W>
+hipError_t launch_pointwise_segs(PointwiseKind kind, int pb, const PwSeg<W> *segs, int nseg,
+                                 int64_t blocks, size_t lds, int grid, const ModConst &mc,
+                                 hipStream_t s)
+{
+    if (nseg <= 0 || blocks <= 0)
+        return hipSuccess;
+    if (grid <= 0 || grid > blocks)
+        grid = (int)(blocks < (1 << 30) ? blocks : (1 << 30));
+    if (kind == PW_PLAIN) {
+        hipLaunchKernelGGL(pw_plain_segs<W>, dim3((unsigned)grid), dim3(kThreads), 0, s, segs, nseg,
+                           blocks, mc);
+        return hipGetLastError();
+    }
+    constexpr int L = sizeof(W) == 8 ? 8 : 4;
+    if (mc.limbs != L || lds > (size_t)kLdsBudget)
+        return hipErrorInvalidValue;
+    switch (pb) {
+    case 8:
+        return mfma_segs_pb<W, L, 8>(segs, nseg, blocks, lds, grid, mc, s);
+    case 4:
+        return mfma_segs_pb<W, L, 4>(segs, nseg, blocks, lds, grid, mc, s);
+    case 2:
+        return mfma_segs_pb<W, L, 2>(segs, nseg, blocks, lds, grid, mc, s);
+    case 1:
+        return mfma_segs_pb<W, L, 1>(segs, nseg, blocks, lds, grid, mc, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_pointwise_segs<uint64_t>(PointwiseKind, int, const PwSeg<uint64_t> *,
+                                                    int, int64_t, size_t, int, const ModConst &,
+                                                    hipStream_t);
+template hipError_t launch_pointwise_segs<uint32_t>(PointwiseKind, int, const PwSeg<uint32_t> *,
+                                                    int, int64_t, size_t, int, const ModConst &,
+                                                    hipStream_t);
 
 }  // namespace pmlg
