@@ -149,6 +149,10 @@ pmlg_status pmlg_batch_plan_create(pmlg_batch_plan *plan, const pmlg_mul_desc *d
 pmlg_status pmlg_batch_plan_run(pmlg_batch_plan plan, void *stream);
 /* Waits for the plan's own stream. */
 pmlg_status pmlg_batch_plan_sync(pmlg_batch_plan plan);
+/* Eager runs with HIP events between the stages on the plan's stream:
+ * ms[0] forward transforms, ms[1] direct convolutions and pointwise
+ * products, ms[2] inverse transforms (averages over `reps` runs). */
+pmlg_status pmlg_batch_plan_profile(pmlg_batch_plan plan, int reps, double *ms);
 /* Configuration chosen for product i (-1 for an empty product). */
 pmlg_status pmlg_batch_plan_config(pmlg_batch_plan plan, int64_t i, int *config);
 /* Kernel launches per run and arena bytes. */
@@ -171,6 +175,41 @@ int pmlg_choose_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t le
  * the context, and returns it in *best_config. */
 pmlg_status pmlg_autotune(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA,
                           int64_t lenB, int reps, double *ms_per_config, int *best_config);
+
+/* ------------------------------------------------------------------------
+ * Learned dispatch (no reference counterpart: the reference dispatches by
+ * hand-set thresholds, mul.c:52-96 and PMBASIS_THRES).  A small fp32 MLP
+ * maps the features (m/64, k/64, n/64, log2(lenA + lenB - 1)/16, bits(p)/64)
+ * through three hidden ReLU layers of 64 to one predicted log(ms) per
+ * configuration; training, gradients and inference run on the device.
+ * ------------------------------------------------------------------------ */
+#define PMLG_MLP_IN 5
+#define PMLG_MLP_HIDDEN 64
+#define PMLG_MLP_OUT 8 /* = pmlg_num_configs() */
+#define PMLG_MLP_PARAMS 9224
+typedef struct pmlg_mlp_s *pmlg_mlp;
+
+/* He-uniform initial weights from `seed`, zero biases. */
+pmlg_status pmlg_mlp_create(pmlg_mlp *mlp, int device, uint64_t seed);
+void pmlg_mlp_destroy(pmlg_mlp mlp);
+/* Flat parameters: for each layer l, W_l (in x out, row-major) then b_l. */
+pmlg_status pmlg_mlp_get_params(pmlg_mlp mlp, float *params);
+pmlg_status pmlg_mlp_set_params(pmlg_mlp mlp, const float *params);
+/* out[r][c] = predicted log(ms) of configuration c for feature row r (host arrays). */
+pmlg_status pmlg_mlp_forward(pmlg_mlp mlp, const float *X, int64_t rows, float *out);
+/* Masked mean-squared error over the rows and its gradient (PMLG_MLP_PARAMS floats). */
+pmlg_status pmlg_mlp_loss_grad(pmlg_mlp mlp, const float *X, const float *Y, const float *mask,
+                               int64_t rows, float *loss, float *grads);
+/* Adam on shuffled mini-batches (rows resident on the device); *final_loss =
+ * masked MSE of the final parameters over the whole set. */
+pmlg_status pmlg_mlp_train(pmlg_mlp mlp, const float *X, const float *Y, const float *mask,
+                           int64_t rows, int epochs, int batch, float lr, uint64_t seed,
+                           float *final_loss);
+/* Feature row of a shape. */
+void pmlg_mlp_features(int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB, uint64_t p,
+                       float *x);
+/* Model consulted by PMLG_METHOD_AUTO (NULL: none; the caller keeps it alive). */
+pmlg_status pmlg_dispatch_set_model(pmlg_mlp mlp);
 
 /* Per-stage device timing (HIP events around each launch).  Stages:
  * 0 forward NTT, 1 pointwise product, 2 inverse NTT (+ merge). */

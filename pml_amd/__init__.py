@@ -102,6 +102,8 @@ def lib():
         L.pmlg_batch_plan_run.argtypes = [vp, vp]
         L.pmlg_batch_plan_sync.restype = st
         L.pmlg_batch_plan_sync.argtypes = [vp]
+        L.pmlg_batch_plan_profile.restype = st
+        L.pmlg_batch_plan_profile.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_double)]
         L.pmlg_batch_plan_config.restype = st
         L.pmlg_batch_plan_config.argtypes = [vp, i64, ctypes.POINTER(c_int)]
         L.pmlg_batch_plan_stats.restype = st
@@ -337,6 +339,12 @@ class BatchPlan:
 
     def sync(self):
         _check(lib().pmlg_batch_plan_sync(self.handle))
+
+    def profile(self, reps=5):
+        """Average ms of the forward, pointwise and inverse stages (eager runs)."""
+        ms = (ctypes.c_double * 3)()
+        _check(lib().pmlg_batch_plan_profile(self.handle, reps, ms))
+        return list(ms)
 
     def config(self, i):
         c = ctypes.c_int()

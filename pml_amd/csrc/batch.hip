@@ -353,12 +353,23 @@ pmlg_status launch_inv(pmlg_batch_plan plan, hipStream_t s)
     return PMLG_OK;
 }
 
-pmlg_status launch_all(pmlg_batch_plan plan, hipStream_t s)
+// stage timing of an eager run: events between the stages (forward
+// transforms, direct convolutions + fills + pointwise, inverse transforms)
+struct StageEvents {
+    hipEvent_t ev[4];
+    bool on;
+};
+
+pmlg_status launch_all(pmlg_batch_plan plan, hipStream_t s, StageEvents *se = nullptr)
 {
     const unsigned char *tb = static_cast<const unsigned char *>(plan->tables);
     pmlg_status st;
+    if (se)
+        HIP_TRY(hipEventRecord(se->ev[0], s), "hipEventRecord");
     if ((st = launch_groups<uint64_t>(plan, s)) != PMLG_OK || (st = launch_groups<uint32_t>(plan, s)) != PMLG_OK)
         return st;
+    if (se)
+        HIP_TRY(hipEventRecord(se->ev[1], s), "hipEventRecord");
     for (const SchoolGroup &g : plan->school) {
         const int grid = (int)std::min<int64_t>(g.blocks, 2048);
         hipLaunchKernelGGL(school_segs, dim3(grid), dim3(kSchoolThreads), 0, s,
@@ -373,8 +384,12 @@ pmlg_status launch_all(pmlg_batch_plan plan, hipStream_t s)
     }
     if ((st = launch_pw<uint64_t>(plan, s)) != PMLG_OK || (st = launch_pw<uint32_t>(plan, s)) != PMLG_OK)
         return st;
+    if (se)
+        HIP_TRY(hipEventRecord(se->ev[2], s), "hipEventRecord");
     if ((st = launch_inv<uint64_t>(plan, s)) != PMLG_OK || (st = launch_inv<uint32_t>(plan, s)) != PMLG_OK)
         return st;
+    if (se)
+        HIP_TRY(hipEventRecord(se->ev[3], s), "hipEventRecord");
     return PMLG_OK;
 }
 
@@ -797,6 +812,39 @@ pmlg_status pmlg_batch_plan_sync(pmlg_batch_plan plan)
         return fail(PMLG_ERR_ARG, "null plan");
     DeviceGuard g(plan->device);
     HIP_TRY(hipStreamSynchronize(plan->stream), "hipStreamSynchronize");
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_batch_plan_profile(pmlg_batch_plan plan, int reps, double *ms)
+{
+    if (!plan || reps <= 0 || !ms)
+        return fail(PMLG_ERR_ARG, "batch_plan_profile: bad arguments");
+    DeviceGuard g(plan->device);
+    StageEvents se{};
+    for (int i = 0; i < 4; ++i)
+        HIP_TRY(hipEventCreate(&se.ev[i]), "hipEventCreate");
+    struct Destroy {
+        StageEvents &e;
+        ~Destroy()
+        {
+            for (int i = 0; i < 4; ++i)
+                (void)hipEventDestroy(e.ev[i]);
+        }
+    } d{se};
+    double acc[3] = {0, 0, 0};
+    for (int r = 0; r < reps; ++r) {
+        pmlg_status st = launch_all(plan, plan->stream, &se);
+        if (st != PMLG_OK)
+            return st;
+        HIP_TRY(hipEventSynchronize(se.ev[3]), "hipEventSynchronize");
+        for (int i = 0; i < 3; ++i) {
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, se.ev[i], se.ev[i + 1]), "hipEventElapsedTime");
+            acc[i] += t;
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+        ms[i] = acc[i] / reps;
     return PMLG_OK;
 }
 

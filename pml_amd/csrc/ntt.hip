@@ -304,14 +304,27 @@ __device__ __forceinline__ uint64_t shoup4(uint64_t x, uint64_t w, uint64_t wq, 
     return ((uint64_t)hi << 32) | (uint32_t)r;
 }
 
+// a + c reduced from [0, 8p) to [0, 4p): the candidate s - 4p is formed
+// first and selected, so no carry chain feeds the select
+__device__ __forceinline__ uint64_t add_4p(uint64_t a, uint64_t c, uint64_t p4, uint64_t np4)
+{
+    const uint64_t s = a + c;
+    const uint64_t t = s + np4;
+    return s >= p4 ? t : s;
+}
+
+// a - c + 4p in (0, 8p) as a + ~c + (4p + 1): two 64-bit adds, no borrow chain
+__device__ __forceinline__ uint64_t sub_4p(uint64_t a, uint64_t c, uint64_t p4)
+{
+    return a + ~c + (p4 + 1);
+}
+
 template <typename W, bool LOOSE>
 __device__ __forceinline__ void bfly_dif(W &a, W &c, W w, W wq, const Bfly<W> &k)
 {
     if constexpr (LOOSE) {
-        W s = a + c;
-        s = s >= k.p4 ? s + k.np4 : s;
-        const W d = a + k.p4 - c;
-        a = s;
+        const W d = sub_4p(a, c, k.p4);
+        a = add_4p(a, c, k.p4, k.np4);
         c = shoup4(d, w, wq, k.np);
     } else {
         const W u = a, v = c;
@@ -320,16 +333,32 @@ __device__ __forceinline__ void bfly_dif(W &a, W &c, W w, W wq, const Bfly<W> &k
     }
 }
 
+// DIF butterfly whose twiddle is 1: (a + c, a - c), both kept in [0, 4p)
+// (loose) or [0, 2p) (strict)
+template <typename W, bool LOOSE>
+__device__ __forceinline__ void bfly_dif1(W &a, W &c, const Bfly<W> &k)
+{
+    if constexpr (LOOSE) {
+        W d = sub_4p(a, c, k.p4);
+        d = d >= k.p4 ? d + k.np4 : d;
+        a = add_4p(a, c, k.p4, k.np4);
+        c = d;
+    } else {
+        const W u = a, v = c;
+        a = add_2p<W>(u, v, k.p2);
+        const W d = u + k.p2 - v;
+        c = d >= k.p2 ? d - k.p2 : d;
+    }
+}
+
 template <typename W, bool LOOSE>
 __device__ __forceinline__ void bfly_dit(W &a, W &c, W w, W wq, const Bfly<W> &k)
 {
     if constexpr (LOOSE) {
         const W v = shoup4(c, w, wq, k.np);
-        W s = a + v;
-        s = s >= k.p4 ? s + k.np4 : s;
-        W d = a + k.p4 - v;
+        W d = sub_4p(a, v, k.p4);
         d = d >= k.p4 ? d + k.np4 : d;
-        a = s;
+        a = add_4p(a, v, k.p4, k.np4);
         c = d;
     } else {
         const W u = a;
@@ -421,7 +450,12 @@ __device__ __forceinline__ void fwd_stages(W (&x)[1 << LOGE], const W (&tw)[2 * 
 #pragma unroll
             for (int jh = 0; jh < (1 << (R - 1 - LB)); ++jh) {
                 const int j = (jh << (LB + 1)) | jl;
-                bfly_dif<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
+                // in the last pass (BLO = 0) the stage-table index is jl itself,
+                // and entry 0 of every stage table is w^0 = 1
+                if (PS::BLO == 0 && jl == 0)
+                    bfly_dif1<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], k);
+                else
+                    bfly_dif<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
             }
         }
     }
@@ -444,7 +478,10 @@ __device__ __forceinline__ void inv_stages(W (&x)[1 << LOGE], const W (&tw)[2 * 
 #pragma unroll
             for (int jh = 0; jh < (1 << (R - 1 - LB)); ++jh) {
                 const int j = (jh << (LB + 1)) | jl;
-                bfly_dit<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
+                if (PS::BLO == 0 && jl == 0)  // twiddle w^0 = 1 (see fwd_stages)
+                    bfly_dif1<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], k);
+                else
+                    bfly_dit<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
             }
         }
     }
@@ -817,6 +854,9 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
 // ===========================================================================
 // Segment-table launches (batched products): same workgroup bodies, the
 // operand matrix of each workgroup looked up by a binary search on block0.
+// Blocks go through the same XCD-aware order as the direct launches (when
+// the grid is a multiple of 8), so that neighbouring entries, which share
+// cache lines in the blocked layout, are handled on one XCD.
 // ===========================================================================
 namespace {
 
@@ -853,10 +893,10 @@ __global__ void __launch_bounds__(kThreads)
 ntt_fwd_segs_lds(const FwdSeg<Win, W> *__restrict__ segs, int nseg, int logN, int logR, int lognb,
                  int logP, NttTab<W> tab, W p)
 {
-    const FwdSeg<Win, W> &sg = seg_of(segs, nseg, blockIdx.x);
+    const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
+    const FwdSeg<Win, W> &sg = seg_of(segs, nseg, b);
     fwd_lds_body<Win, W>(sg.in, sg.in_stride, sg.in_len, sg.out, sg.count << logR,
-                         ((int64_t)blockIdx.x - sg.block0) << lognb, logN, logR, lognb, logP, tab,
-                         p);
+                         (b - sg.block0) << lognb, logN, logR, lognb, logP, tab, p);
 }
 
 template <typename W, typename Wout>
@@ -864,10 +904,10 @@ __global__ void __launch_bounds__(kThreads)
 ntt_inv_segs_lds(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logN, int logR,
                  int lognb, int logP, NttTab<W> tab, W p)
 {
-    const InvSeg<W, Wout> &sg = seg_of(segs, nseg, blockIdx.x);
+    const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
+    const InvSeg<W, Wout> &sg = seg_of(segs, nseg, b);
     inv_lds_body<W, Wout>(sg.in, sg.out, sg.out_stride, sg.lo, sg.hi, sg.tmp, sg.count << logR,
-                          ((int64_t)blockIdx.x - sg.block0) << lognb, logN, logR, lognb, logP,
-                          tab, p);
+                          (b - sg.block0) << lognb, logN, logR, lognb, logP, tab, p);
 }
 
 template <typename Win, typename W, bool LOOSE, int L, int LOGE>
@@ -875,9 +915,10 @@ __global__ void __launch_bounds__(1 << (L - LOGE))
 ntt_fwd_segs_reg(const FwdSeg<Win, W> *__restrict__ segs, int nseg, int logR, int logP,
                  NttTab<W> tab, W p)
 {
-    const FwdSeg<Win, W> &sg = seg_of(segs, nseg, blockIdx.x);
+    const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
+    const FwdSeg<Win, W> &sg = seg_of(segs, nseg, b);
     fwd_reg_body<Win, W, LOOSE, L, LOGE>(sg.in, sg.in_stride, sg.in_len, sg.out, sg.count << logR,
-                                         (int64_t)blockIdx.x - sg.block0, logR, logP, tab, p);
+                                         b - sg.block0, logR, logP, tab, p);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
@@ -885,10 +926,10 @@ __global__ void __launch_bounds__(1 << (L - LOGE))
 ntt_inv_segs_reg(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logR, int logP,
                  NttTab<W> tab, W p)
 {
-    const InvSeg<W, Wout> &sg = seg_of(segs, nseg, blockIdx.x);
+    const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
+    const InvSeg<W, Wout> &sg = seg_of(segs, nseg, b);
     inv_reg_body<W, Wout, LOOSE, L, LOGE>(sg.in, sg.out, sg.out_stride, sg.lo, sg.hi, sg.tmp,
-                                          sg.count << logR, (int64_t)blockIdx.x - sg.block0, logR,
-                                          logP, tab, p);
+                                          sg.count << logR, b - sg.block0, logR, logP, tab, p);
 }
 
 template <typename W, typename Wout>

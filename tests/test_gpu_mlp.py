@@ -1,0 +1,139 @@
+"""Device MLP of the learned dispatch: forward pass and gradients against the
+fp64 restatement (oracle/mlp.py), training on the device, and the dispatch
+contract (the network only chooses among exact algorithms: products stay
+bit-exact whatever it picks)."""
+import numpy as np
+import pytest
+
+from oracle import mlp as R
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P60 = 1152921504556515329
+P30 = 998244353
+
+
+@pytest.fixture(scope="module")
+def learn():
+    from pml_amd import learn
+    return learn
+
+
+def _data(rng, rows):
+    X = rng.uniform(0, 1, size=(rows, 5)).astype(np.float32)
+    Y = rng.normal(size=(rows, 8)).astype(np.float32)
+    M = (rng.uniform(size=(rows, 8)) < 0.7).astype(np.float32)
+    return X, Y, M
+
+
+def test_forward_matches_fp64(learn):
+    rng = np.random.default_rng(1)
+    mlp = learn.Mlp(seed=3)
+    theta = mlp.params.astype(np.float64)
+    X, _, _ = _data(rng, 300)
+    got = mlp.forward(X)
+    ref, _ = R.forward(theta, X.astype(np.float64))
+    # fp32 accumulation over at most 64 terms per unit
+    assert np.allclose(got, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+    mlp.close()
+
+
+@pytest.mark.parametrize("rows", [1, 37, 256])
+def test_gradients_match_fp64_oracle(learn, rows):
+    rng = np.random.default_rng(rows)
+    mlp = learn.Mlp(seed=5)
+    theta = rng.normal(scale=0.25, size=R.n_params())
+    mlp.params = theta
+    theta32 = mlp.params.astype(np.float64)
+    X, Y, M = _data(rng, rows)
+    M[0, 0] = 1.0  # at least one valid entry
+    loss, g = mlp.loss_grad(X, Y, M)
+    L_ref, g_ref = R.loss_grad(theta32, X.astype(np.float64), Y.astype(np.float64),
+                               M.astype(np.float64))
+    assert abs(loss - L_ref) <= 1e-4 * max(1.0, abs(L_ref))
+    err = np.linalg.norm(g - g_ref) / max(np.linalg.norm(g_ref), 1e-30)
+    assert err < 1e-4, err
+    # elementwise on the large components
+    big = np.abs(g_ref) > 1e-3 * np.abs(g_ref).max()
+    assert np.allclose(g[big], g_ref[big], rtol=2e-3, atol=1e-6 * np.abs(g_ref).max())
+    mlp.close()
+
+
+def test_training_fits_a_teacher(learn):
+    """Targets from a fixed random network: device training (Adam, shuffled
+    batches) must drive the masked MSE far below its starting value."""
+    rng = np.random.default_rng(9)
+    teacher = rng.normal(scale=0.3, size=R.n_params())
+    X = rng.uniform(0, 1, size=(4096, 5))
+    Y, _ = R.forward(teacher, X)
+    M = (rng.uniform(size=Y.shape) < 0.8).astype(np.float32)
+    mlp = learn.Mlp(seed=2)
+    start, _ = mlp.loss_grad(X, Y, M)
+    final = mlp.train(X, Y, M, epochs=60, batch=256, lr=2e-3)
+    assert final < 0.05 * start, (start, final)
+    # the reported loss is the loss of the parameters left on the device
+    again, _ = mlp.loss_grad(X, Y, M)
+    assert abs(again - final) <= 1e-3 * max(final, 1e-6) + 1e-6
+    mlp.close()
+
+
+def test_learned_dispatch_picks_argmin_and_stays_exact(learn):
+    import pml_amd
+    rng = np.random.default_rng(12)
+    mlp = learn.Mlp(seed=21)
+    mlp.params = rng.normal(scale=0.5, size=R.n_params())
+    ctxs = {P60: pml_amd.Context(P60), P30: pml_amd.Context(P30)}
+    try:
+        mlp.install()
+        shapes = [(P60, 5, 7, 3, 100, 90), (P30, 9, 2, 4, 300, 17), (P60, 2, 40, 2, 33, 600),
+                  (P30, 12, 12, 12, 64, 64), (P60, 3, 3, 3, 1500, 1200)]
+        items, seen = [], set()
+        for (p, m, k, n, la, lb) in shapes:
+            x = learn.features(m, k, n, la, lb, p)
+            pred = mlp.forward(x[None, :])[0]
+            valid = [c for c in range(8) if pml_amd.config_valid(ctxs[p], m, k, n, la, lb, c)]
+            want = min(valid, key=lambda c: pred[c])
+            got = pml_amd.choose_config(ctxs[p], m, k, n, la, lb, "auto")
+            assert got == want
+            seen.add(got)
+            items.append((ctxs[p], O.rand_uniform((m, k, la), p, la), O.rand_uniform((k, n, lb), p, lb)))
+        res = pml_amd.mul_batch(items, method="auto")
+        for (ctx, A, B), C in zip(items, res):
+            assert np.array_equal(C, O.polmat_mul(A, B, ctx.p))
+        # the threshold rule is still available beside the model
+        for (p, m, k, n, la, lb) in shapes:
+            assert pml_amd.choose_config(ctxs[p], m, k, n, la, lb, "threshold") in (0, 1, 2)
+    finally:
+        learn.Mlp.uninstall()
+        mlp.close()
+        for c in ctxs.values():
+            c.close()
+
+
+def test_learned_dispatch_gains_over_threshold_on_unseen_shapes(learn):
+    """The gain test: trained on the device on 20,000 recorded shapes of the
+    uneven-batch workload, the network's picks on the 2,000 held-out shapes
+    cost less recorded time than the threshold rule's and come close to the
+    per-shape best.  (Outputs are exact under any pick: see the test above.)"""
+    import pml_amd
+    shapes, T = learn.load_timings()
+    X, Y, M = learn.dataset(shapes[:20000], T[:20000])
+    ctx_by_p = {P30: pml_amd.Context(P30), P60: pml_amd.Context(P60)}
+    try:
+        mlp = learn.Mlp(seed=11)
+        mlp.train(X, Y, M, epochs=300, batch=256, lr=1e-3, seed=7)
+        ev = learn.evaluate(mlp, shapes[20000:], T[20000:], ctx_by_p)
+        print("held-out: learned %.1f ms, threshold %.1f ms, best %.1f ms, gain %.3f" % (
+            ev["learned"], ev["threshold"], ev["best"], ev["gain_vs_threshold"]))
+        assert ev["shapes"] == 2000
+        assert ev["gain_vs_threshold"] > 1.04
+        assert ev["learned"] <= 1.03 * ev["best"]
+        mlp.close()
+        packaged = learn.Mlp.load(learn.DEFAULT_MODEL)
+        ev2 = learn.evaluate(packaged, shapes[20000:], T[20000:], ctx_by_p)
+        assert ev2["gain_vs_threshold"] > 1.04
+        packaged.close()
+    finally:
+        for c in ctx_by_p.values():
+            c.close()
