@@ -161,6 +161,10 @@ void pmlg_batch_plan_destroy(pmlg_batch_plan plan);
 /* One-shot: plan, run, synchronise, destroy. */
 pmlg_status pmlg_polmat_mul_batch(const pmlg_mul_desc *descs, int64_t count, void *stream);
 
+/* n residues in [0, p) from a SplitMix64 stream of `seed`, written on the
+ * device (synthetic inputs of large workloads without a host round trip). */
+pmlg_status pmlg_fill_random(pmlg_ctx ctx, uint64_t *dst, int64_t n, uint64_t seed, void *stream);
+
 /* Configurations: 0 direct convolution; 1..7 transforms with the plain or
  * MFMA pointwise kernel and different transform radices / MFMA block sizes. */
 int pmlg_num_configs(void);

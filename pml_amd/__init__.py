@@ -112,6 +112,8 @@ def lib():
         L.pmlg_batch_plan_destroy.argtypes = [vp]
         L.pmlg_polmat_mul_batch.restype = st
         L.pmlg_polmat_mul_batch.argtypes = [ctypes.POINTER(MulDesc), i64, vp]
+        L.pmlg_fill_random.restype = st
+        L.pmlg_fill_random.argtypes = [vp, vp, i64, u64, vp]
         L.pmlg_num_configs.restype = c_int
         L.pmlg_num_configs.argtypes = []
         L.pmlg_config_valid.restype = c_int
@@ -246,6 +248,10 @@ class Context:
             _check(lib().pmlg_polmat_mulmid(self.handle, _p(out), _p(A), _p(B), m, k, n, la, lb,
                                             nlo, nhi, _method(method)))
         return out
+
+    def fill_random(self, dst, n, seed, stream=None):
+        """n residues in [0, p) from a seeded stream, generated on the device."""
+        _check(lib().pmlg_fill_random(self.handle, int(dst), int(n), int(seed), stream))
 
     def dev_copy(self, dst, src, nbytes, stream=None):
         _check(lib().pmlg_dev_copy(self.handle, int(dst), int(src), int(nbytes), stream))

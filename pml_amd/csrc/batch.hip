@@ -908,6 +908,22 @@ pmlg_status pmlg_autotune(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t
                         best_config);
 }
 
+pmlg_status pmlg_fill_random(pmlg_ctx ctx, uint64_t *dst, int64_t n, uint64_t seed, void *stream)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (n < 0 || (n > 0 && !dst))
+        return fail(PMLG_ERR_ARG, "fill_random: bad arguments");
+    if (n == 0)
+        return PMLG_OK;
+    DeviceGuard g(ctx->device);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipLaunchKernelGGL(fill_random_kernel, dim3(grid_for(n)), dim3(256), 0, s, dst, n, ctx->p, seed);
+    HIP_TRY(hipGetLastError(), "fill_random");
+    return PMLG_OK;
+}
+
 int pmlg_num_configs(void)
 {
     return kNumConfigs;

@@ -1,0 +1,143 @@
+"""Timing harness for the approximant-basis and uneven-batch workloads.
+
+    python -m pml_amd.perf pmbasis [--reps 3]
+        PM-Basis of a 64x32 matrix, order 2048, zero shift, p = 998244353
+        (inputs: SplitMix64 residues of seed 401, the golden fixture's seed).
+    python -m pml_amd.perf batch [--pairs 512] [--method auto|threshold|plain|mfma]
+        512 independent products, m, k, n uniform in 2..64, lengths
+        log-uniform in 16..16384, moduli 998244353 / 2^60-3*2^24+1 (50/50),
+        inputs generated on the device, one batch plan replayed as a graph.
+
+Each prints one JSON line.  The outputs are checked on the device where the
+check is cheap (the PM-Basis residual P*F = 0 mod x^order; the batch's small
+products against a second configuration).
+"""
+import argparse
+import ctypes
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import BatchPlan, Context, DeviceArray, MulDesc, _check, lib, learn
+
+P30 = 998244353
+P60 = 1152921504556515329
+
+
+def run_pmbasis(a):
+    m, n, order = 64, 32, 2048
+    ctx = Context(P30)
+    rng = np.random.default_rng(401)
+    F = rng.integers(0, P30, size=(m, n, order), dtype=np.uint64)
+    shift = np.zeros(m, dtype=np.int64)
+    P, sh = ctx.pmbasis(F, order, shift)  # warm-up (tables, arena)
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        P, sh = ctx.pmbasis(F, order, shift)
+    dt = (time.perf_counter() - t0) / a.reps
+    # residual P*F mod x^order on the device
+    R = ctx.mulmid(P, F, 0, order)
+    out = {"workload": "pmbasis 64x32 order 2048 p=998244353 shift 0", "seconds_per_basis": round(dt, 5),
+           "bases_per_s": round(1.0 / dt, 3), "basis_length": int(P.shape[2]),
+           "shift_out_sum": int(sh.sum()), "residual_zero": bool(not R.any()), "reps": a.reps}
+    print(json.dumps(out))
+    return 0
+
+
+def run_batch(a):
+    shapes = learn.draw_shapes(a.pairs, a.seed)
+    ctxs = {P30: Context(P30), P60: Context(P60)}
+    model = learn.install_default() if a.method == "auto" else None
+    meth = {"auto": 0, "plain": 1, "mfma": 2, "threshold": 3}[a.method]
+    descs = (MulDesc * len(shapes))()
+    bufs = []
+    tot_bytes = 0
+    tot_macs = 0
+    for i, (p, m, k, n, la, lb) in enumerate(shapes):
+        ctx = ctxs[p]
+        lc = la + lb - 1
+        dA, dB, dC = DeviceArray(ctx, m * k * la * 8), DeviceArray(ctx, k * n * lb * 8), \
+            DeviceArray(ctx, m * n * lc * 8)
+        ctx.fill_random(dA.ptr, m * k * la, 1000 + 2 * i)
+        ctx.fill_random(dB.ptr, k * n * lb, 1001 + 2 * i)
+        bufs.append((dA, dB, dC))
+        d = descs[i]
+        d.ctx, d.A, d.B, d.C = ctx.handle, dA.ptr, dB.ptr, dC.ptr
+        d.m, d.k, d.n, d.lenA, d.lenB, d.method, d.config = m, k, n, la, lb, meth, -1
+        tot_bytes += 8 * (m * k * la + k * n * lb + m * n * lc)
+        tot_macs += (1 << math.ceil(math.log2(lc))) * m * k * n
+    for c in ctxs.values():
+        c.sync()
+    h = ctypes.c_void_p()
+    t0 = time.perf_counter()
+    _check(lib().pmlg_batch_plan_create(ctypes.byref(h), descs, len(shapes), 1))
+    plan_s = time.perf_counter() - t0
+    plan = h.value
+    _check(lib().pmlg_batch_plan_run(plan, None))  # eager pass + capture
+    _check(lib().pmlg_batch_plan_sync(plan))
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        _check(lib().pmlg_batch_plan_run(plan, None))
+    _check(lib().pmlg_batch_plan_sync(plan))
+    dt = (time.perf_counter() - t0) / a.reps
+    launches, arena = ctypes.c_int64(), ctypes.c_uint64()
+    _check(lib().pmlg_batch_plan_stats(plan, ctypes.byref(launches), ctypes.byref(arena)))
+    cfg = np.zeros(8, dtype=np.int64)
+    for i in range(len(shapes)):
+        c = ctypes.c_int()
+        _check(lib().pmlg_batch_plan_config(plan, i, ctypes.byref(c)))
+        if c.value >= 0:
+            cfg[c.value] += 1
+    # check: the smallest products again under a different configuration (plain
+    # transforms), compared on the host
+    order = sorted(range(len(shapes)), key=lambda i: np.prod(shapes[i][1:4]) * (shapes[i][4] + shapes[i][5]))
+    sel = order[: a.check]
+    ok = True
+    for i in sel:
+        p, m, k, n, la, lb = shapes[i]
+        dA, dB, dC = bufs[i]
+        lc = la + lb - 1
+        C = np.zeros((m, n, lc), dtype=np.uint64)
+        dC.download(C)
+        A = np.zeros((m, k, la), dtype=np.uint64)
+        B = np.zeros((k, n, lb), dtype=np.uint64)
+        dA.download(A)
+        dB.download(B)
+        C2 = ctxs[p].mul(A, B, method="plain")
+        ok = ok and bool(np.array_equal(C, C2))
+    lib().pmlg_batch_plan_destroy(plan)
+    out = {"workload": "batch of %d uneven products (C4 distribution, seed %d)" % (a.pairs, a.seed),
+           "method": a.method, "seconds_per_batch": round(dt, 5),
+           "products_per_s": round(a.pairs / dt, 1), "algorithmic_GB_per_s": round(tot_bytes / dt / 1e9, 1),
+           "pointwise_modmacs_per_s": float("%.4g" % (tot_macs / dt)), "launches_per_batch": launches.value,
+           "arena_GB": round(arena.value / 1e9, 2), "plan_seconds": round(plan_s, 3),
+           "configs": cfg.tolist(), "checked_products": len(sel), "check_ok": ok, "reps": a.reps}
+    print(json.dumps(out))
+    for b in bufs:
+        for x in b:
+            x.free()
+    if model:
+        model.close()
+    return 0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m pml_amd.perf")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    p = sub.add_parser("pmbasis")
+    p.add_argument("--reps", type=int, default=3)
+    b = sub.add_parser("batch")
+    b.add_argument("--pairs", type=int, default=512)
+    b.add_argument("--seed", type=int, default=4)
+    b.add_argument("--reps", type=int, default=3)
+    b.add_argument("--check", type=int, default=24)
+    b.add_argument("--method", default="auto", choices=["auto", "threshold", "plain", "mfma"])
+    a = ap.parse_args(argv)
+    return run_pmbasis(a) if a.cmd == "pmbasis" else run_batch(a)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
