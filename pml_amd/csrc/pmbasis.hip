@@ -27,6 +27,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <climits>
+
 #include "ctx.hpp"
 #include "host_arith.hpp"
 
@@ -90,6 +92,60 @@ __device__ uint64_t invmod_g(uint64_t a, const ModConst &mc)
     return r;
 }
 
+// Arithmetic of the step kernel.  Values in LDS are kept in the policy's
+// representation: plain residues (Plain), or Montgomery residues x 2^32 mod p
+// for p < 2^30 (Mont32: one 32x32 -> 64 product, one low product and one
+// multiply-add per reduction instead of a 64-bit Barrett step).  Zero maps to
+// zero either way, so pivot tests read the representation directly.
+struct PlainArith {
+    using V = uint64_t;  // word held in LDS
+    ModConst mc;
+    __device__ explicit PlainArith(const ModConst &c) : mc(c) {}
+    __device__ V to(uint64_t x) const { return x; }
+    __device__ uint64_t from(V x) const { return x; }
+    __device__ V mul(V a, V b) const { return mulmod_g(a, b, mc); }
+    __device__ V sub(V a, V b) const { return submod_g(a, b, mc.p); }
+    __device__ V inv(V a) const { return invmod_g(a, mc); }
+};
+
+struct Mont32Arith {
+    using V = uint32_t;
+    uint32_t p, pinv, r1, r2;  // pinv = -p^-1 mod 2^32, r1 = 2^32 mod p, r2 = 2^64 mod p
+    __device__ explicit Mont32Arith(const ModConst &c)
+        : p((uint32_t)c.p), pinv((uint32_t)c.pinvneg), r1((uint32_t)c.c32), r2((uint32_t)c.c64) {}
+    __device__ V redc(uint64_t t) const
+    {
+        // t < p^2 < 2^60: t + m p < 2^60 + 2^62, and the quotient u is below 2p;
+        // min(u, u - p) picks u - p exactly when u >= p (u - p wraps otherwise)
+        const uint32_t mm = (uint32_t)t * pinv;
+        const uint32_t u = (uint32_t)((t + (uint64_t)mm * p) >> 32);
+        const uint32_t v = u - p;
+        return v < u ? v : u;
+    }
+    __device__ V mul(V a, V b) const { return redc((uint64_t)a * b); }
+    __device__ V sub(V a, V b) const
+    {
+        // a - b, plus p when it wraps: min(d, d + p) as unsigned words
+        const uint32_t d = a - b, e = d + p;
+        return e < d ? e : d;
+    }
+    __device__ V to(uint64_t x) const { return mul((V)x, r2); }
+    __device__ uint64_t from(V x) const { return redc(x); }
+    __device__ V inv(V a) const
+    {
+        // Fermat a^(p-2), in the Montgomery domain (r1 is 1 there)
+        uint32_t e = p - 2;
+        V r = r1, b = a;
+        while (e) {
+            if (e & 1)
+                r = mul(r, b);
+            b = mul(b, b);
+            e >>= 1;
+        }
+        return r;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // kernel 1: permutation, left nullspace, step description
 // ---------------------------------------------------------------------------
@@ -98,39 +154,56 @@ __device__ uint64_t invmod_g(uint64_t a, const ModConst &mc)
 // column); for a pivot at (pr, c) every other row r becomes
 // pv * row_r - At[r][c] * row_pr with pv = At[pr][c].  The k-th row of the
 // reduced row echelon form is row prow[k] divided by its pivot.
+template <typename Arith>
 __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, int ord)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    using V = typename Arith::V;
     const int m = a.m, n = a.n;
-    uint64_t *At = reinterpret_cast<uint64_t *>(smem);           // n x m
-    uint64_t *fcol = At + (size_t)n * m;                         // kMaxDim
-    uint64_t *inv = fcol + kMaxDim;                              // kMaxDim
+    const Arith ar(a.mc);
+    const int ms = m | 1;                   // odd row stride of At
+    const int rstep = kStepThreads / m;     // m <= kMaxDim <= kStepThreads
+    const int rpt = (n + rstep - 1) / rstep;  // rows of At per thread
+    const int npad = rpt * rstep;           // rows n .. npad-1 are zero
+    // 64-bit words first (sh), then the V arrays, then the int arrays
+    int64_t *sh = reinterpret_cast<int64_t *>(smem);             // kMaxDim
+    V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // npad x ms
+    V *fcol = At + npad * ms;                                    // 2 x kMaxDim: column snapshots
+    V *inv = fcol + 2 * kMaxDim;                                 // kMaxDim
     int *perm = reinterpret_cast<int *>(inv + kMaxDim);          // kMaxDim
     int *pivcol = perm + kMaxDim;                                // kMaxDim
     int *plist = pivcol + kMaxDim;                               // kMaxDim
     int *prow = plist + kMaxDim;                                 // kMaxDim
-    int *ispiv = prow + kMaxDim;                                 // kMaxDim
-    int64_t *sh = reinterpret_cast<int64_t *>(ispiv + kMaxDim);  // kMaxDim
-    __shared__ int s_state[4];
+    int *ispiv = prow + kMaxDim;                                 // kMaxDim (rows of At)
+    int *colpiv = ispiv + kMaxDim;                               // kMaxDim (columns of At)
+    int *pos = colpiv + kMaxDim;                                 // kMaxDim
+    int *cflag = pos + kMaxDim;                                  // 2 x kMaxDim: pivot candidates
+    __shared__ int s_state[2];
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const uint64_t p = a.mc.p;
 
     if (tid == 0) {
+        // control words: independent loads first, then the updates
         int *c = a.ctl;
-        if (c[C_FLIP]) {
-            c[C_CUR] ^= 1;
-            c[C_FLIP] = 0;
-        }
-        c[C_LEN] += c[C_GROW];
+        const int flip = c[C_FLIP], cur = c[C_CUR], len = c[C_LEN], grow = c[C_GROW],
+                  done = c[C_DONE];
+        const int ncur = flip ? cur ^ 1 : cur;
+        c[C_CUR] = ncur;
+        c[C_FLIP] = 0;
+        c[C_LEN] = len + grow;
         c[C_GROW] = 0;
-        if (c[C_DONE])
+        if (done)
             c[C_MODE] = 0;
-        s_state[0] = c[C_DONE];
-        s_state[1] = c[C_CUR];
+        s_state[0] = done;
+        s_state[1] = ncur;
     }
-    for (int i = tid; i < m; i += kStepThreads)
+    for (int i = tid; i < m; i += kStepThreads) {
         sh[i] = a.shift[i];
-    for (int r = tid; r < n; r += kStepThreads)
+        pos[i] = 0;
+        colpiv[i] = 0;
+    }
+    for (int r = tid; r < kMaxDim; r += kStepThreads)
         ispiv[r] = 0;
     __syncthreads();
     if (s_state[0])
@@ -138,85 +211,98 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     const int cur = s_state[1];
     const uint64_t *R = a.R[cur];
 
-    // stable permutation sorting (shift, index)
-    for (int i = tid; i < m; i += kStepThreads) {
-        const int64_t si = sh[i];
-        int pos = 0;
-        for (int j = 0; j < m; ++j) {
-            const int64_t sj = sh[j];
-            pos += (sj < si) || (sj == si && j < i);
-        }
-        perm[pos] = i;
+    // stable permutation sorting (shift, index): pos[i] = #{j : (sh_j, j) < (sh_i, i)}
+    for (int idx = tid; idx < m * m; idx += kStepThreads) {
+        const int j = idx / m, i = idx - j * m;  // neighbouring lanes update different counters
+        const int64_t si = sh[i], sj = sh[j];
+        if ((sj < si) || (sj == si && j < i))
+            atomicAdd(&pos[i], 1);
     }
     __syncthreads();
-    // this thread's elements of At: column col, rows r0, r0 + rstep, ...
-    const int rstep = kStepThreads / m;  // m <= kMaxDim <= kStepThreads
+    for (int i = tid; i < m; i += kStepThreads)
+        perm[pos[i]] = i;
+    __syncthreads();
+    // this thread's elements of At: column col, rows r0 + k rstep, k < rpt
     const int col = tid % m;
     const int r0 = tid / m;
     const bool active = r0 < rstep;
     if (active) {
         const int64_t rowbase = (int64_t)perm[col] * n;
-        for (int r = r0; r < n; r += rstep)
-            At[(size_t)r * m + col] = R[(rowbase + r) * a.ordL + ord];
+        for (int k = 0; k < rpt; ++k) {
+            const int r = r0 + k * rstep;
+            const V v = r < n ? ar.to(R[(rowbase + r) * a.ordL + ord]) : (V)0;
+            At[r * ms + col] = v;  // At has npad rows; fcol and cflag only n
+            if (col == 0 && r < n) {
+                fcol[r] = v;
+                cflag[r] = v != 0;
+            }
+        }
     }
     __syncthreads();
 
+    // Gauss-Jordan, one barrier per column.  While eliminating column c the
+    // threads of column c+1 keep its new values (fcol) and flag the rows that
+    // can be its pivot (nonzero, not a pivot row yet); every wave then finds
+    // the first flagged row with a ballot, so no atomics or extra barrier.
     int rank = 0;
     for (int c = 0; c < m && rank < n; ++c) {
-        // wave 0: pivot search among rows that are not pivots yet, snapshot of column c
-        if (tid < 64) {
-            int pr = -1;
-            for (int base = 0; base < n; base += 64) {
-                const int r = base + tid;
-                uint64_t v = 0;
-                if (r < n) {
-                    v = At[(size_t)r * m + c];
-                    fcol[r] = v;
-                }
-                const uint64_t mask = __ballot(r < n && v != 0 && !ispiv[r]);
-                if (mask && pr < 0)
-                    pr = base + __ffsll((unsigned long long)mask) - 1;
-            }
-            if (tid == 0) {
-                s_state[2 + (c & 1)] = pr;
-                if (pr >= 0)
-                    ispiv[pr] = 1;
-            }
-        }
-        __syncthreads();
-        const int pr = s_state[2 + (c & 1)];  // double-buffered: no second barrier needed
-        if (pr < 0)
-            continue;  // uniform
-        if (tid == 0) {
+        const int *cf = cflag + (c & 1) * kMaxDim;
+        int *cfn = cflag + ((c + 1) & 1) * kMaxDim;
+        const V *fc = fcol + (c & 1) * kMaxDim;
+        V *fn = fcol + ((c + 1) & 1) * kMaxDim;
+        const unsigned long long b0 = __ballot(lane < n && cf[lane]);
+        const unsigned long long b1 = __ballot(lane + 64 < n && cf[lane + 64]);
+        const int pr = b0 ? __ffsll(b0) - 1 : (b1 ? 63 + __ffsll(b1) : -1);
+        const bool next = active && col == c + 1;
+        if (pr >= 0) {
+            // identical writes from every lane: no branch on the thread index
             prow[rank] = pr;
             pivcol[rank] = c;
-        }
-        const uint64_t pv = fcol[pr];
-        if (active) {
-            const uint64_t pcol = At[(size_t)pr * m + col];
-#pragma unroll 4
-            for (int r = r0; r < n; r += rstep) {
-                if (r == pr)
-                    continue;
-                uint64_t *x = At + (size_t)r * m + col;
-                *x = submod_g(mulmod_g(pv, *x, a.mc), mulmod_g(fcol[r], pcol, a.mc), p);
+            ispiv[pr] = 1;  // read from the next iteration on (the row is excluded below)
+            colpiv[c] = 1;
+            const V pv = fc[pr];
+            if (active) {
+                const V pcol = At[pr * ms + col];
+                for (int k = 0; k < rpt; ++k) {
+                    const int r = r0 + k * rstep;
+                    const int rr = r < n ? r : n - 1;  // padding rows (zero) read a valid slot
+                    V *x = At + r * ms + col;
+                    const V v = *x;
+                    const V f = r < n ? fc[rr] : (V)0;
+                    const V nv = ar.sub(ar.mul(pv, v), ar.mul(f, pcol));
+                    const V w = r != pr ? nv : v;
+                    *x = w;
+                    if (next && r < n) {
+                        fn[r] = w;
+                        cfn[r] = w != 0 && r != pr && !ispiv[r];
+                    }
+                }
+            }
+            ++rank;
+        } else if (next) {
+            for (int k = 0; k < rpt; ++k) {
+                const int r = r0 + k * rstep;
+                if (r >= n)
+                    break;
+                const V v = At[r * ms + col];
+                fn[r] = v;
+                cfn[r] = v != 0 && !ispiv[r];
             }
         }
-        ++rank;
         __syncthreads();
     }
     // pivots of the reduced echelon form: 1 / At[prow[k]][pivcol[k]]
     for (int k = tid; k < rank; k += kStepThreads)
-        inv[k] = invmod_g(At[(size_t)prow[k] * m + pivcol[k]], a.mc);
-    if (tid == 0) {
-        // permutation list of left_nullspace_compact: pivots, then non-pivots
-        int q = 0, np = rank;
-        for (int i = 0; i < m; ++i) {
-            if (q < rank && pivcol[q] == i)
-                plist[q++] = i;
-            else
-                plist[np++] = i;
-        }
+        inv[k] = ar.inv(At[prow[k] * ms + pivcol[k]]);
+    // permutation list of left_nullspace_compact: pivot columns, then the others
+    for (int i = tid; i < m; i += kStepThreads) {
+        int before = 0;
+        for (int j = 0; j < i; ++j)
+            before += colpiv[j];
+        if (colpiv[i])
+            plist[before] = i;
+        else
+            plist[rank + i - before] = i;
     }
     __syncthreads();
     const int nullity = m - rank;
@@ -224,30 +310,31 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     if (mode == 1) {
         for (int idx = tid; idx < nullity * rank; idx += kStepThreads) {
             const int t = idx / rank, r = idx - t * rank;
-            const uint64_t v = mulmod_g(At[(size_t)prow[r] * m + plist[rank + t]], inv[r], a.mc);
+            const uint64_t v = ar.from(ar.mul(At[prow[r] * ms + plist[rank + t]], inv[r]));
             a.X[(size_t)t * m + r] = v ? p - v : 0;
         }
         for (int i = tid; i < m; i += kStepThreads) {
             const int row = perm[plist[i]];
             a.piv[i] = row;
             a.rowinfo[row] = i < rank ? -1 - i : i - rank;
+            if (i < rank)
+                a.shift[row] = sh[row] + 1;
         }
-        for (int i = tid; i < rank; i += kStepThreads)
-            a.shift[perm[plist[i]]] += 1;
     } else if (mode == 2) {
         for (int i = tid; i < m; i += kStepThreads)
-            a.shift[i] += a.ordL - ord;
+            a.shift[i] = sh[i] + (a.ordL - ord);
     }
     if (tid == 0) {
         int *c = a.ctl;
+        const int len = c[C_LEN];
         c[C_MODE] = mode;
         c[C_RANK] = rank;
         if (mode != 0)
             c[C_FLIP] = 1;
         if (mode == 2) {
             c[C_SHIFT] = a.ordL - ord;
-            c[C_LOLD] = c[C_LEN];
-            c[C_LEN] += a.ordL - ord;
+            c[C_LOLD] = len;
+            c[C_LEN] = len + (a.ordL - ord);
             c[C_DONE] = 1;
         }
     }
@@ -256,35 +343,61 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
 // ---------------------------------------------------------------------------
 // kernel 2: apply the step to appbas (and to the residual series)
 // ---------------------------------------------------------------------------
-// base + sum_r X[t][r] * src[row piv[r], column j, coefficient d]
+// base + sum_r X[t][r] * src[row piv[r], column j, coefficient d].  The
+// loads of a group are issued before its products so that they overlap.
 template <bool SMALL>
 __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArgs &a, int t, int rank,
                                               int stride, int cols, int j, int d, uint64_t base)
 {
     const uint64_t *xr = a.X + (size_t)t * a.m;
+    const uint64_t *sj = src + (size_t)j * stride + d;
+    const size_t rstride = (size_t)cols * stride;
     if constexpr (SMALL) {
-        // p < 2^30: products < 2^60, 15 of them plus a residue fit in 64 bits
+        // p < 2^30: a product is below 2^60, 8 of them plus a residue below 2^64
+        constexpr int G = 8;
         uint64_t acc = base;
-        int cnt = 0;
-        for (int r = 0; r < rank; ++r) {
-            acc += xr[r] * src[((size_t)a.piv[r] * cols + j) * stride + d];
-            if (++cnt == 15) {
-                acc = barrett(acc, a.mc.bm, a.mc.p);
-                cnt = 0;
+        int r = 0;
+        for (; r + G <= rank; r += G) {
+            uint64_t sv[G], xv[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                sv[u] = sj[(size_t)a.piv[r + u] * rstride];
+                xv[u] = xr[r + u];
             }
+            uint64_t part = 0;
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                part += xv[u] * sv[u];
+            acc = barrett(acc + part, a.mc.bm, a.mc.p);
         }
-        return barrett(acc, a.mc.bm, a.mc.p);
+        uint64_t part = 0;
+        for (; r < rank; ++r)
+            part += xr[r] * sj[(size_t)a.piv[r] * rstride];
+        return barrett(acc + part, a.mc.bm, a.mc.p);
     } else {
+        // 64-bit words: exact 128-bit sums, folded every 15 products (each < 2^124)
+        constexpr int G = 5;
         uint64_t hi = 0, lo = base;
-        int cnt = 0;
-        for (int r = 0; r < rank; ++r) {
-            mac128(hi, lo, xr[r], src[((size_t)a.piv[r] * cols + j) * stride + d]);
-            if (++cnt == 15) {
+        int r = 0, cnt = 0;
+        for (; r + G <= rank; r += G) {
+            uint64_t sv[G], xv[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                sv[u] = sj[(size_t)a.piv[r + u] * rstride];
+                xv[u] = xr[r + u];
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                mac128(hi, lo, xv[u], sv[u]);
+            cnt += G;
+            if (cnt >= 15) {
                 lo = reduce128(hi, lo, a.mc);
                 hi = 0;
                 cnt = 0;
             }
         }
+        for (; r < rank; ++r)
+            mac128(hi, lo, xr[r], sj[(size_t)a.piv[r] * rstride]);
         return reduce128(hi, lo, a.mc);
     }
 }
@@ -448,24 +561,36 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     hipLaunchKernelGGL(leaf_init_kernel, dim3(grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL)),
                        dim3(256), 0, s, a, F, strideF, lt);
     HIP_TRY(hipGetLastError(), "leaf_init_kernel");
-    const size_t lds = ((size_t)n * m + 3 * kMaxDim) * 8 + 5 * kMaxDim * sizeof(int);
+    // LDS of the step kernel: shifts, At padded to a multiple of the row step,
+    // column snapshots and inverses, then 9 int arrays (sizes as in the kernel)
+    const int rstep = kStepThreads / m;
+    const size_t npad = (size_t)((n + rstep - 1) / rstep) * rstep;
+    const size_t lds = (npad * (m | 1) + 4 * kMaxDim) * 8 + 9 * kMaxDim * sizeof(int);
     static bool attr = false;
     if (!attr) {
-        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(((size_t)kMaxDim * kMaxDim + 3 * kMaxDim) * 8 +
-                                          5 * kMaxDim * sizeof(int))),
+        const size_t npmax = kMaxDim + kStepThreads / kMaxDim - 1;
+        const int maxlds = (int)((npmax * (kMaxDim + 1) + 4 * kMaxDim) * 8 +
+                                 9 * kMaxDim * sizeof(int));
+        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel<PlainArith>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, maxlds),
+                "hipFuncSetAttribute(mbasis_step_kernel)");
+        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel<Mont32Arith>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, maxlds),
                 "hipFuncSetAttribute(mbasis_step_kernel)");
         attr = true;
     }
     const unsigned ugrid = grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL);
     const bool small = ctx->p < ((uint64_t)1 << 30);
     for (int ord = 0; ord < ordL; ++ord) {
-        hipLaunchKernelGGL(mbasis_step_kernel, dim3(1), dim3(kStepThreads), lds, s, a, ord);
-        if (small)
+        if (small) {
+            hipLaunchKernelGGL(mbasis_step_kernel<Mont32Arith>, dim3(1), dim3(kStepThreads), lds, s, a,
+                               ord);
             hipLaunchKernelGGL(mbasis_update_kernel<true>, dim3(ugrid), dim3(256), 0, s, a, ord);
-        else
+        } else {
+            hipLaunchKernelGGL(mbasis_step_kernel<PlainArith>, dim3(1), dim3(kStepThreads), lds, s, a,
+                               ord);
             hipLaunchKernelGGL(mbasis_update_kernel<false>, dim3(ugrid), dim3(256), 0, s, a, ord);
+        }
     }
     HIP_TRY(hipGetLastError(), "mbasis kernels");
     int ctl[C_WORDS];
