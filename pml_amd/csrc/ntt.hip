@@ -17,6 +17,10 @@ namespace pmlg {
 namespace {
 
 constexpr int kThreads = 256;
+#ifndef PMLG_INV_WAVES
+#define PMLG_INV_WAVES 4
+#endif
+constexpr int kInvWaves = PMLG_INV_WAVES;  // minimum waves per SIMD of the inverse register kernels
 
 // Point-blocked evaluation layout: position `pos` of entry `e` (of `count`
 // entries) lives at ((pos >> logP) * count + e) * P + (pos mod P), P = 2^logP,
@@ -498,14 +502,75 @@ __device__ __forceinline__ void fwd_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 <<
 {
     using PS = PassShape<L, LOGE, S0>;
     fwd_stages<W, LOOSE, L, LOGE, S0, PS::R - 1>(x, tw, k);
-    if constexpr (S0 + PS::R < L)
-        load_pass_tw<W, L, LOGE, S0 + PS::R>(tw, t, stw);  // in flight during the exchange
-    lds_store<W, L, LOGE, S0>(x, lds, t);
-    __syncthreads();
     if constexpr (S0 + PS::R < L) {
+        load_pass_tw<W, L, LOGE, S0 + PS::R>(tw, t, stw);  // in flight during the exchange
+        lds_store<W, L, LOGE, S0>(x, lds, t);
+        __syncthreads();
         lds_load<W, L, LOGE, S0 + PS::R>(x, lds, t);
         __syncthreads();
         fwd_passes<W, LOOSE, L, LOGE, S0 + PS::R>(x, tw, lds, t, stw, k);
+    }
+}
+
+// first stage of the last forward pass (it ends at stage L)
+template <int L, int LOGE>
+constexpr int last_pass_start()
+{
+    return ((L - 1) / LOGE) * LOGE;
+}
+
+// CNT consecutive words (CNT a power of two, 16-byte aligned runs when
+// CNT * sizeof(W) >= 16) moved with vector loads / stores
+template <typename W, int CNT>
+__device__ __forceinline__ void store_words(W *dst, const W *v)
+{
+    if constexpr (CNT * sizeof(W) >= 16) {
+        constexpr int PER = 16 / sizeof(W);
+#pragma unroll
+        for (int q = 0; q < CNT / PER; ++q) {
+            uint4 u;
+            if constexpr (sizeof(W) == 8) {
+                u.x = (uint32_t)v[2 * q];
+                u.y = (uint32_t)(v[2 * q] >> 32);
+                u.z = (uint32_t)v[2 * q + 1];
+                u.w = (uint32_t)(v[2 * q + 1] >> 32);
+            } else {
+                u.x = v[4 * q];
+                u.y = v[4 * q + 1];
+                u.z = v[4 * q + 2];
+                u.w = v[4 * q + 3];
+            }
+            reinterpret_cast<uint4 *>(dst)[q] = u;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CNT; ++j)
+            dst[j] = v[j];
+    }
+}
+
+template <typename W, int CNT>
+__device__ __forceinline__ void load_words(const W *src, W *v)
+{
+    if constexpr (CNT * sizeof(W) >= 16) {
+        constexpr int PER = 16 / sizeof(W);
+#pragma unroll
+        for (int q = 0; q < CNT / PER; ++q) {
+            const uint4 u = reinterpret_cast<const uint4 *>(src)[q];
+            if constexpr (sizeof(W) == 8) {
+                v[2 * q] = ((uint64_t)u.y << 32) | u.x;
+                v[2 * q + 1] = ((uint64_t)u.w << 32) | u.z;
+            } else {
+                v[4 * q] = u.x;
+                v[4 * q + 1] = u.y;
+                v[4 * q + 2] = u.z;
+                v[4 * q + 3] = u.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CNT; ++j)
+            v[j] = src[j];
     }
 }
 
@@ -534,7 +599,7 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
-    constexpr int E = 1 << LOGE, T = 1 << (L - LOGE), M = 1 << L;
+    constexpr int E = 1 << LOGE, M = 1 << L;
     const int t = threadIdx.x;
     const int64_t e = u >> logR;
     const int R = 1 << logR;
@@ -571,11 +636,31 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
     }
     const Bfly<W> k(p);
     fwd_passes<W, LOOSE, L, LOGE, 0>(x, tw, lds, t, tab.stw, k);
+    // the last pass leaves each thread runs of 2^R consecutive positions:
+    // store them from registers (no LDS round trip)
     const int64_t count = units >> logR;
+    using PSL = PassShape<L, LOGE, last_pass_start<L, LOGE>()>;
+    constexpr int RUN = 1 << PSL::R;
+    // words per vector store: within one layout block of 2^logP positions
+    // (logP >= 2, and for 32-bit words logP = 3 unless N < 8)
+    constexpr int VEC = RUN < 4 ? RUN : 4;
 #pragma unroll
-    for (int v = 0; v < E; ++v) {
-        const int idx = t + T * v;
-        out[blk_addr(e, (int64_t)gg * M + idx, count, logP)] = bfly_final<W, LOOSE>(lds[swz(idx)], k);
+    for (int g = 0; g < PSL::G; ++g) {
+        W y[RUN];
+#pragma unroll
+        for (int j = 0; j < RUN; ++j)
+            y[j] = bfly_final<W, LOOSE>(x[g * RUN + j], k);
+        if (logP >= 2) {
+#pragma unroll
+            for (int j0 = 0; j0 < RUN; j0 += VEC)
+                store_words<W, VEC>(out + blk_addr(e, (int64_t)gg * M + PSL::index(t, g, j0), count,
+                                                   logP),
+                                    y + j0);
+        } else {
+#pragma unroll
+            for (int j = 0; j < RUN; ++j)
+                out[blk_addr(e, (int64_t)gg * M + PSL::index(t, g, j), count, logP)] = y[j];
+        }
     }
 }
 
@@ -596,7 +681,7 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
-    constexpr int E = 1 << LOGE, T = 1 << (L - LOGE), M = 1 << L;
+    constexpr int E = 1 << LOGE, M = 1 << L;
     const int t = threadIdx.x;
     const int64_t e = u >> logR;
     const int R = 1 << logR;
@@ -606,14 +691,26 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
     constexpr int SLAST = ((L - 1) / LOGE) * LOGE;
     W tw[2 * (E - 1)];
     load_pass_tw<W, L, LOGE, SLAST>(tw, t, tab.sitw);
-#pragma unroll
-    for (int v = 0; v < E; ++v) {
-        const int idx = t + T * v;
-        lds[swz(idx)] = in[blk_addr(e, (int64_t)gg * M + idx, count, logP)];
-    }
-    __syncthreads();
+    // the first inverse pass (stages SLAST..L-1) works on runs of 2^R
+    // consecutive positions per thread: load them straight from global memory
+    using PSF = PassShape<L, LOGE, SLAST>;
+    constexpr int RUN = 1 << PSF::R;
+    constexpr int VEC = RUN < 4 ? RUN : 4;
     W x[E];
-    lds_load<W, L, LOGE, SLAST>(x, lds, t);
+#pragma unroll
+    for (int g = 0; g < PSF::G; ++g) {
+        if (logP >= 2) {
+#pragma unroll
+            for (int j0 = 0; j0 < RUN; j0 += VEC)
+                load_words<W, VEC>(in + blk_addr(e, (int64_t)gg * M + PSF::index(t, g, j0), count,
+                                                 logP),
+                                   x + g * RUN + j0);
+        } else {
+#pragma unroll
+            for (int j = 0; j < RUN; ++j)
+                x[g * RUN + j] = in[blk_addr(e, (int64_t)gg * M + PSF::index(t, g, j), count, logP)];
+        }
+    }
     const Bfly<W> k(p);
     inv_passes<W, LOOSE, L, LOGE, SLAST>(x, tw, lds, t, tab.sitw, k);
     // pass-0 arrangement: element j is index (j << (L-LOGE)) | t
@@ -633,8 +730,10 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
     }
 }
 
+// With 16 elements per thread the inverse is held to 128 VGPRs, so that four
+// waves share each SIMD and hide the LDS-exchange latency.
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
-__global__ void __launch_bounds__(1 << (L - LOGE))
+__global__ void __launch_bounds__(1 << (L - LOGE)) __attribute__((amdgpu_waves_per_eu(LOGE == 4 ? kInvWaves : 1)))
 ntt_inv_reg(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
             int64_t hi, W *__restrict__ tmp, int64_t units, int logR, int logP, NttTab<W> tab, W p)
 {
@@ -922,7 +1021,7 @@ ntt_fwd_segs_reg(const FwdSeg<Win, W> *__restrict__ segs, int nseg, int logR, in
 }
 
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
-__global__ void __launch_bounds__(1 << (L - LOGE))
+__global__ void __launch_bounds__(1 << (L - LOGE)) __attribute__((amdgpu_waves_per_eu(LOGE == 4 ? kInvWaves : 1)))
 ntt_inv_segs_reg(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logR, int logP,
                  NttTab<W> tab, W p)
 {
