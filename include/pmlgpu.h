@@ -24,9 +24,12 @@
  *   pmlg_polmat_mul_batch: no single reference counterpart; it runs many
  *                          independent nmod_poly_mat_multiply calls at once.
  *
- * Supported moduli: primes p < 2^62 with 2^ceil(log2(len)) | p-1 for the
- * transform length len the call needs (at most 2^15 for p >= 2^30, 2^16
- * for p < 2^30).  Others return PMLG_ERR_UNSUPPORTED.
+ * Supported moduli: any p in [2, 2^62).  When 2^ceil(log2(N)) | p-1 for the
+ * transform length N a product needs (N <= 2^15 for p >= 2^30, 2^16 for
+ * p < 2^30), the product is evaluated modulo p itself; otherwise (non-FFT
+ * primes, composite moduli) it is computed modulo three 61-bit FFT primes and
+ * recombined by CRT (N <= 2^15).  Approximant bases need an odd prime.
+ * Longer products return PMLG_ERR_UNSUPPORTED.
  */
 #ifndef PMLGPU_H
 #define PMLGPU_H
@@ -59,7 +62,8 @@ typedef struct pmlg_ctx_s *pmlg_ctx;
  * buffers on one device.  Calls on one context are serialised internally. */
 pmlg_status pmlg_ctx_create(pmlg_ctx *ctx, uint64_t p, int device);
 void pmlg_ctx_destroy(pmlg_ctx ctx);
-/* log2 of the largest transform length usable with this prime */
+/* log2 of the largest transform length evaluated modulo p itself (-1 when
+ * every product takes the multi-prime path, e.g. for a composite modulus) */
 int pmlg_ctx_max_log_len(pmlg_ctx ctx);
 /* Stream used by the host-pointer calls (a hipStream_t). */
 void *pmlg_ctx_stream(pmlg_ctx ctx);

@@ -322,10 +322,13 @@ pmlg_status mulmid_dev_strided(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride,
     // cyclic length: no wrapped term may land in [nlo, hi_eff)
     int64_t need = hi_eff > lc - nlo ? hi_eff : lc - nlo;
     const int logN = ceil_log2(need);
-    if (logN > ctx->max_log)
-        return fail(PMLG_ERR_UNSUPPORTED,
-                    "transform length 2^" + std::to_string(logN) +
-                        " not supported by this prime (2-adic valuation or size limit)");
+    if (logN > ctx->max_log) {
+        // no 2^logN-th root of unity mod p (or p composite): multi-prime path
+        if (hi_eff < nhi)
+            HIP_TRY(fill_zero(dOut, m * n * out_stride, s), "fill_zero");
+        return crt_product(ctx, dOut, out_stride, dA, strideA, dB, strideB, m, k, n, la, lb, nlo,
+                           hi_eff, method, s);
+    }
     PointwiseKind kind;
     pmlg_status st = pick_kind(ctx, method, m, k, n, &kind);
     if (st != PMLG_OK)
@@ -365,8 +368,8 @@ pmlg_status pmlg_ctx_create(pmlg_ctx *out, uint64_t p, int device)
     if (!out)
         return fail(PMLG_ERR_ARG, "null output pointer");
     *out = nullptr;
-    if (p < 3 || p >= ((uint64_t)1 << 62) || !host_is_prime(p))
-        return fail(PMLG_ERR_UNSUPPORTED, "modulus must be an odd prime below 2^62");
+    if (p < 2 || p >= ((uint64_t)1 << 62))
+        return fail(PMLG_ERR_UNSUPPORTED, "modulus must be in [2, 2^62)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(PMLG_ERR_DEVICE, "no HIP device available");
@@ -378,11 +381,18 @@ pmlg_status pmlg_ctx_create(pmlg_ctx *out, uint64_t p, int device)
     ctx->p = p;
     ctx->device = device;
     ctx->small = p < ((uint64_t)1 << 30);
-    int v2 = __builtin_ctzll(p - 1);
-    const int cap = ctx->small ? NttLimits<uint32_t>::max_log_m + 2
-                               : NttLimits<uint64_t>::max_log_m + 2;
-    ctx->max_log = v2 < cap ? v2 : cap;
-    ctx->root = host_root_of_unity_2k(p, ctx->max_log);
+    ctx->is_prime = p > 2 && host_is_prime(p);
+    if (ctx->is_prime) {
+        int v2 = __builtin_ctzll(p - 1);
+        const int cap = ctx->small ? NttLimits<uint32_t>::max_log_m + 2
+                                   : NttLimits<uint64_t>::max_log_m + 2;
+        ctx->max_log = v2 < cap ? v2 : cap;
+        ctx->root = host_root_of_unity_2k(p, ctx->max_log);
+    } else {
+        // no transform modulo p itself: every product takes the multi-prime path
+        ctx->max_log = -1;
+        ctx->root = 0;
+    }
     ctx->mc = host_modconst(p);
     // radix of the register transforms (autotuned values may replace these)
     if (const char *v = std::getenv("PMLG_NTT_FWD_LOGE"))
@@ -403,6 +413,11 @@ void pmlg_ctx_destroy(pmlg_ctx ctx)
 {
     if (!ctx)
         return;
+    for (pmlg_ctx &c : ctx->crt)
+        if (c) {
+            pmlg_ctx_destroy(c);
+            c = nullptr;
+        }
     {
         DeviceGuard g(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
@@ -423,6 +438,8 @@ void pmlg_ctx_destroy(pmlg_ctx ctx)
         ctx->arena.release();
         ctx->tune_io.release();
         ctx->tune_ws.release();
+        ctx->crt_in.release();
+        ctx->crt_out.release();
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

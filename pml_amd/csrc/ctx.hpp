@@ -102,6 +102,11 @@ struct pmlg_ctx_s {
     std::mutex tune_mu;
     std::map<uint64_t, int> tuned;
     pmlg::DevBuf tune_io, tune_ws;
+    // moduli without the roots a product needs: products modulo three FFT
+    // primes (contexts created on first use) recombined by CRT
+    bool is_prime = true;
+    pmlg_ctx crt[3] = {nullptr, nullptr, nullptr};
+    pmlg::DevBuf crt_in, crt_out;
 };
 
 namespace pmlg {
@@ -114,6 +119,14 @@ hipError_t fill_zero(uint64_t *p, int64_t n, hipStream_t s);
 pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
                               const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t lenA,
                               int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s);
+// Coefficients [nlo, nhi) of A*B for a modulus whose roots of unity do not
+// reach the transform length (or a composite modulus): the exact integer
+// product modulo three FFT primes, recombined (Garner) and reduced mod p.
+// Lengths already clipped as in mulmid_dev_strided; ctx->mu held.
+pmlg_status crt_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const uint64_t *dA,
+                        int64_t strideA, const uint64_t *dB, int64_t strideB, int64_t m, int64_t k,
+                        int64_t n, int64_t la, int64_t lb, int64_t nlo, int64_t hi, int method,
+                        hipStream_t s);
 // Same with explicit operand strides (entry e of A at dA + e*strideA, length lenA <= strideA).
 pmlg_status mulmid_dev_strided(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride,
                                const uint64_t *dA, int64_t strideA, const uint64_t *dB,

@@ -673,6 +673,8 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
 {
     if (m <= 0 || n <= 0 || m > kMaxDim || n > kMaxDim || order < 0 || lenF < 0)
         return fail(PMLG_ERR_ARG, "approximant basis: dimensions must be in [1, 128], order >= 0");
+    if (!ctx->is_prime)
+        return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
     if (capP < order + 1)
         return fail(PMLG_ERR_ARG, "approximant basis: output capacity must be at least order + 1");
     // device arena

@@ -1,8 +1,11 @@
 """Timing harness for the approximant-basis and uneven-batch workloads.
 
-    python -m pml_amd.perf pmbasis [--reps 3]
-        PM-Basis of a 64x32 matrix, order 2048, zero shift, p = 998244353
-        (inputs: SplitMix64 residues of seed 401, the golden fixture's seed).
+    python -m pml_amd.perf pmbasis [--reps 3] [--p P]
+        PM-Basis of a 64x32 matrix, order 2048, zero shift, p = 998244353 by
+        default (any prime: non-FFT primes multiply through the CRT path).
+    python -m pml_amd.perf mul [--p P]
+        32x32 by 32x32 products of length 4096 modulo p, device-resident
+        (the reference's published benchmark uses p = 723501827743829161).
     python -m pml_amd.perf batch [--pairs 512] [--method auto|threshold|plain|mfma]
         512 independent products, m, k, n uniform in 2..64, lengths
         log-uniform in 16..16384, moduli 998244353 / 2^60-3*2^24+1 (50/50),
@@ -27,11 +30,36 @@ P30 = 998244353
 P60 = 1152921504556515329
 
 
+def run_mul(a):
+    """One 32x32 by 32x32 product of length 4096 per call, device-resident."""
+    p = a.p
+    ctx = Context(p)
+    M = K = N = 32
+    L = 4096
+    dA, dB = DeviceArray(ctx, M * K * L * 8), DeviceArray(ctx, K * N * L * 8)
+    dC = DeviceArray(ctx, M * N * (2 * L - 1) * 8)
+    ctx.fill_random(dA.ptr, M * K * L, 1)
+    ctx.fill_random(dB.ptr, K * N * L, 2)
+    ctx.mul_dev(dC.ptr, dA.ptr, dB.ptr, M, K, N, L, L)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        ctx.mul_dev(dC.ptr, dA.ptr, dB.ptr, M, K, N, L, L)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / a.reps
+    out = {"workload": "polmat_mul 32x32x32 len 4096 p=%d" % p,
+           "path": "transforms mod p" if ctx.max_log_len >= 13 else "3-prime CRT",
+           "seconds_per_product": round(dt, 6), "products_per_s": round(1.0 / dt, 1), "reps": a.reps}
+    print(json.dumps(out))
+    return 0
+
+
 def run_pmbasis(a):
     m, n, order = 64, 32, 2048
-    ctx = Context(P30)
+    p = a.p
+    ctx = Context(p)
     rng = np.random.default_rng(401)
-    F = rng.integers(0, P30, size=(m, n, order), dtype=np.uint64)
+    F = rng.integers(0, p, size=(m, n, order), dtype=np.uint64)
     shift = np.zeros(m, dtype=np.int64)
     P, sh = ctx.pmbasis(F, order, shift)  # warm-up (tables, arena)
     t0 = time.perf_counter()
@@ -40,7 +68,7 @@ def run_pmbasis(a):
     dt = (time.perf_counter() - t0) / a.reps
     # residual P*F mod x^order on the device
     R = ctx.mulmid(P, F, 0, order)
-    out = {"workload": "pmbasis 64x32 order 2048 p=998244353 shift 0", "seconds_per_basis": round(dt, 5),
+    out = {"workload": "pmbasis 64x32 order 2048 p=%d shift 0" % p, "seconds_per_basis": round(dt, 5),
            "bases_per_s": round(1.0 / dt, 3), "basis_length": int(P.shape[2]),
            "shift_out_sum": int(sh.sum()), "residual_zero": bool(not R.any()), "reps": a.reps}
     print(json.dumps(out))
@@ -129,6 +157,10 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     p = sub.add_parser("pmbasis")
     p.add_argument("--reps", type=int, default=3)
+    p.add_argument("--p", type=int, default=P30, help="modulus (default 998244353)")
+    mu = sub.add_parser("mul")
+    mu.add_argument("--reps", type=int, default=20)
+    mu.add_argument("--p", type=int, default=P60, help="modulus (default 2^60-3*2^24+1)")
     b = sub.add_parser("batch")
     b.add_argument("--pairs", type=int, default=512)
     b.add_argument("--seed", type=int, default=4)
@@ -136,7 +168,11 @@ def main(argv=None):
     b.add_argument("--check", type=int, default=24)
     b.add_argument("--method", default="auto", choices=["auto", "threshold", "plain", "mfma"])
     a = ap.parse_args(argv)
-    return run_pmbasis(a) if a.cmd == "pmbasis" else run_batch(a)
+    if a.cmd == "pmbasis":
+        return run_pmbasis(a)
+    if a.cmd == "mul":
+        return run_mul(a)
+    return run_batch(a)
 
 
 if __name__ == "__main__":

@@ -21,6 +21,7 @@
 
 static const uint64_t P60 = 1152921504556515329ULL;
 static const uint64_t P30 = 998244353ULL;
+static const uint64_t P_NOFFT = 723501827743829161ULL; /* no 2^k roots: multi-prime path */
 static uint64_t g_state = 12345;
 
 static void fail(const char *what)
@@ -261,6 +262,7 @@ int main(void)
     check_mul(nmod_poly_mat_mul_vandermonde1, "nmod_poly_mat_mul_vandermonde1", 6, 5, 4, 200, 1, 0, P60);
     check_mul(nmod_poly_mat_mul_vandermonde2, "nmod_poly_mat_mul_vandermonde2", 6, 5, 4, 200, 1, 0, P30);
     check_mul(nmod_poly_mat_mul_waksman, "nmod_poly_mat_mul_waksman", 6, 5, 4, 200, 1, 0, P30);
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 6, 5, 7, 400, 1, 1, P_NOFFT);
     check_mul_alias(6, 100, P60);
     check_mulmid(7, 5, 6, 200, 300, 100, 300, 0, P60);
     check_mulmid(7, 5, 6, 200, 300, 0, 1000, 0, P60); /* nhi beyond the product length */
@@ -273,6 +275,7 @@ int main(void)
     check_basis(0, 5, 5, 64, 64, 1, P30); /* square F */
     check_basis(1, 4, 2, 40, 40, 2, P30);
     check_basis(1, 7, 3, 25, 25, 0, P60);
+    check_basis(0, 5, 2, 90, 90, 1, P_NOFFT);
     printf("all drop-in checks passed\n");
     return 0;
 }
