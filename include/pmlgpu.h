@@ -113,6 +113,35 @@ pmlg_status pmlg_pmbasis_dev(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *
                              int64_t *shift, const uint64_t *dF, int64_t m, int64_t n,
                              int64_t lenF, int64_t order, int method, void *stream);
 
+/* Minimal left kernel bases (nmod_poly_mat_kernel.h:91-105): a basis of
+ * {u : u F = 0} in shift-ordered weak Popov form (pivots increasing), for F
+ * m x n with padded length lenF.
+ *   algo PMLG_KERNEL_ZLS          replaces nmod_poly_mat_kernel_zls_approx
+ *                                 (kernel.c:232-478, Zhou-Labahn-Storjohann);
+ *   algo PMLG_KERNEL_VIA_APPROX   replaces nmod_poly_mat_kernel_via_approx
+ *                                 (kernel.c:148-228, one approximant basis).
+ * K receives m x m entries of capacity capK (entry (i,j) coefficient d at
+ * K[(i*m + j)*capK + d]); its first *nullity rows form the basis, *lenK is
+ * their normalised length.  pivind and shift have m entries: on return the
+ * first *nullity hold the pivot indices and the shifted row degrees of the
+ * basis rows (shift holds the input shift on entry).  Every approximant
+ * basis, product and middle product runs on the device.  When capK < *lenK,
+ * K is not written and PMLG_ERR_ARG is returned with *lenK and *nullity set:
+ * call again with capK >= *lenK.  1 <= m <= 128; n <= 128 for
+ * PMLG_KERNEL_VIA_APPROX; the modulus must be an odd prime. */
+#define PMLG_KERNEL_ZLS 0
+#define PMLG_KERNEL_VIA_APPROX 1
+pmlg_status pmlg_kernel_basis(pmlg_ctx ctx, uint64_t *K, int64_t capK, int64_t *lenK,
+                              int64_t *nullity, int64_t *pivind, int64_t *shift, const uint64_t *F,
+                              int64_t m, int64_t n, int64_t lenF, int algo, int method);
+/* Same with F and K in device memory, enqueued on `stream` (NULL: the
+ * context's); synchronises it before returning (the recursion reads degrees
+ * back at every level). */
+pmlg_status pmlg_kernel_basis_dev(pmlg_ctx ctx, uint64_t *dK, int64_t capK, int64_t *lenK,
+                                  int64_t *nullity, int64_t *pivind, int64_t *shift,
+                                  const uint64_t *dF, int64_t m, int64_t n, int64_t lenF, int algo,
+                                  int method, void *stream);
+
 /* Rank (and determinant when r == c, else 0) of an r x c constant matrix
  * (row-major, host pointer, r*c <= 2^22), by Gaussian elimination on the
  * device.  Used by the approximant-basis certificate; no reference counterpart

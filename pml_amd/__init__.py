@@ -7,9 +7,14 @@ include/pmlgpu.h).  It mirrors the reference's operations
     nmod_poly_mat_mulmid(res, A, B, nlo, nhi)   -> Context.mulmid(A, B, nlo, nhi)
     nmod_poly_mat_pmbasis(appbas, shift, F, d)  -> Context.pmbasis(F, d, shift)
     nmod_poly_mat_mbasis(appbas, shift, F, d)   -> Context.mbasis(F, d, shift)
+    nmod_poly_mat_kernel_zls_approx(ker, piv, shift, F)
+                                                -> Context.kernel_basis(F, shift, "zls")
+    nmod_poly_mat_kernel_via_approx(ker, piv, shift, F)
+                                                -> Context.kernel_basis(F, shift, "via_approx")
 
 (flint-extras/src/nmod_poly_mat_multiply.h:92,122-124;
- flint-extras/src/nmod_poly_mat_approximant.h:282-285,321-324).
+ flint-extras/src/nmod_poly_mat_approximant.h:282-285,321-324;
+ flint-extras/src/nmod_poly_mat_kernel.h:91-105).
 
 Polynomial matrices are numpy uint64 arrays of shape (rows, cols, len),
 coefficients in [0, p).  There is no CPU fallback: if the library is missing
@@ -27,6 +32,7 @@ METHOD_AUTO, METHOD_PLAIN, METHOD_MFMA, METHOD_THRESHOLD = 0, 1, 2, 3
 _METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "mfma": METHOD_MFMA,
             "threshold": METHOD_THRESHOLD}
 BATCH_GRAPH = 1
+KERNEL_ZLS, KERNEL_VIA_APPROX = 0, 1
 
 
 class MulDesc(ctypes.Structure):
@@ -129,6 +135,13 @@ def lib():
                                        c_int]
             L.pmlg_mbasis.restype = st
             L.pmlg_mbasis.argtypes = [vp, _u64p, _i64p, _i64p, _u64p, i64, i64, i64, i64, i64]
+        if hasattr(L, "pmlg_kernel_basis"):
+            L.pmlg_kernel_basis.restype = st
+            L.pmlg_kernel_basis.argtypes = [vp, _u64p, i64, _i64p, _i64p, _i64p, _i64p, _u64p, i64,
+                                            i64, i64, c_int, c_int]
+            L.pmlg_kernel_basis_dev.restype = st
+            L.pmlg_kernel_basis_dev.argtypes = [vp, vp, i64, _i64p, _i64p, _i64p, _i64p, vp, i64,
+                                                i64, i64, c_int, c_int, vp]
         _LIB = L
     return _LIB
 
@@ -280,6 +293,37 @@ class Context:
     def mbasis(self, F, order, shift):
         """Iterative approximant basis (nmod_poly_mat_mbasis)."""
         return self._basis(lib().pmlg_mbasis, F, order, shift)
+
+    def kernel_basis(self, F, shift=None, algo="zls", method="auto", cap=None):
+        """Shift-ordered weak Popov basis of the left kernel of F
+        (nmod_poly_mat_kernel_zls_approx / _via_approx).  Returns (nullity, K,
+        pivind, shift_out): K has nullity rows, shift_out their shifted row
+        degrees.  shift=None takes the row degrees of F (zero rows as 0)."""
+        F = _c(F)
+        m, n, lf = F.shape
+        if shift is None:
+            nz = F != 0
+            deg = np.where(nz.any(axis=2), lf - 1 - np.argmax(nz[..., ::-1], axis=2), -1)
+            shift = np.maximum(deg.max(axis=1), 0) if n else np.zeros(m)
+        sh0 = np.ascontiguousarray(shift, dtype=np.int64).copy()
+        a = {"zls": KERNEL_ZLS, "via_approx": KERNEL_VIA_APPROX}[algo] if isinstance(algo, str) \
+            else int(algo)
+        cap = int(cap) if cap else (min(m, n) + 1) * max(lf, 1) + int(sh0.max() - sh0.min()) + 1
+        while True:
+            sh = sh0.copy()
+            piv = np.zeros(m, dtype=np.int64)
+            K = np.zeros((m, m, cap), dtype=np.uint64)
+            L, nul = ctypes.c_int64(0), ctypes.c_int64(0)
+            status = lib().pmlg_kernel_basis(self.handle, _p(K), cap, ctypes.byref(L),
+                                             ctypes.byref(nul), piv.ctypes.data_as(_i64p),
+                                             sh.ctypes.data_as(_i64p), _p(F), m, n, lf, a,
+                                             _method(method))
+            if status != 0 and L.value > cap:
+                cap = L.value
+                continue
+            _check(status)
+            k = nul.value
+            return k, K[:k, :, :L.value].copy(), piv[:k].copy(), sh[:k].copy()
 
     # -- stage timing ---------------------------------------------------------
     def profile(self, on=True):

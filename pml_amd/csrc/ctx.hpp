@@ -133,4 +133,17 @@ pmlg_status mulmid_dev_strided(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride,
                                int64_t strideB, int64_t m, int64_t k, int64_t n, int64_t lenA,
                                int64_t lenB, int64_t nlo, int64_t nhi, int method, hipStream_t s);
 
+// Approximant basis (PM-Basis, or M-Basis when iterative) of F (entry e at
+// dF + e*strideF, length lenF) into dP (capacity capP >= order + 1) or, when hP
+// is set, into host memory with a normalised length; shift in/out on the host.
+// ctx->mu held.
+pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP,
+                         int64_t *shift, const uint64_t *dF, int64_t strideF, int64_t m, int64_t n,
+                         int64_t lenF, int64_t order, int method, bool iterative, hipStream_t s,
+                         uint64_t *hP);
+// Rank of the r x c matrix of words at dM (row-major, overwritten), with
+// dScratch holding r + 2 words; synchronises s.  ctx->mu held.
+pmlg_status const_rank_dev(pmlg_ctx ctx, uint64_t *dM, uint64_t *dScratch, int64_t r, int64_t c,
+                           int64_t *rank, hipStream_t s);
+
 }  // namespace pmlg

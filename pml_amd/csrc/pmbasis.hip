@@ -666,9 +666,11 @@ size_t arena_bytes(int64_t m, int64_t n, int64_t order)
     return (size_t)((2.5 * lvl + leafb + 4096.0) * 8.0);
 }
 
+}  // namespace
+
 pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP,
-                         int64_t *shift, const uint64_t *dF, int64_t m, int64_t n, int64_t lenF,
-                         int64_t order, int method, bool iterative, hipStream_t s,
+                         int64_t *shift, const uint64_t *dF, int64_t strideF, int64_t m, int64_t n,
+                         int64_t lenF, int64_t order, int method, bool iterative, hipStream_t s,
                          uint64_t *hP)
 {
     if (m <= 0 || n <= 0 || m > kMaxDim || n > kMaxDim || order < 0 || lenF < 0)
@@ -690,9 +692,10 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
     Poly P;
     pmlg_status st;
     if (iterative)
-        st = leaf(ctx, ar, dF, lenF, std::min(lenF, order), (int)m, (int)n, (int)order, dShift, &P, s);
+        st = leaf(ctx, ar, dF, strideF, std::min(lenF, order), (int)m, (int)n, (int)order, dShift, &P,
+                  s);
     else
-        st = pmb_rec(ctx, ar, dF, lenF, lenF, (int)m, (int)n, order, dShift, method, kPmbasisThres,
+        st = pmb_rec(ctx, ar, dF, strideF, lenF, (int)m, (int)n, order, dShift, method, kPmbasisThres,
                      &P, s);
     if (st != PMLG_OK)
         return st;
@@ -725,8 +728,6 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
     return PMLG_OK;
 }
 
-}  // namespace
-
 }  // namespace pmlg
 
 using namespace pmlg;
@@ -743,7 +744,8 @@ pmlg_status pmlg_pmbasis_dev(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *
     std::lock_guard<std::mutex> lk(ctx->mu);
     DeviceGuard g(ctx->device);
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    return basis_common(ctx, dP, capP, lenP, shift, dF, m, n, lenF, order, method, false, s, nullptr);
+    return basis_common(ctx, dP, capP, lenP, shift, dF, lenF, m, n, lenF, order, method, false, s,
+                        nullptr);
 }
 
 static pmlg_status basis_host(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
@@ -763,7 +765,7 @@ static pmlg_status basis_host(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t 
     if (lenF > 0)
         HIP_TRY(hipMemcpyAsync(ctx->in_a.ptr, F, fb, hipMemcpyHostToDevice, s), "H2D(F)");
     return basis_common(ctx, nullptr, capP, lenP, shift, static_cast<const uint64_t *>(ctx->in_a.ptr),
-                        m, n, lenF, order, method, iterative, s, P);
+                        lenF, m, n, lenF, order, method, iterative, s, P);
 }
 
 pmlg_status pmlg_pmbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
@@ -865,6 +867,21 @@ __global__ void eval_kernel(uint64_t *out, const uint64_t *A, int64_t entries, i
 }
 
 }  // namespace
+
+pmlg_status const_rank_dev(pmlg_ctx ctx, uint64_t *dM, uint64_t *dScratch, int64_t r, int64_t c,
+                           int64_t *rank, hipStream_t s)
+{
+    int *drank = reinterpret_cast<int *>(dScratch + r);
+    hipLaunchKernelGGL(const_gauss_kernel, dim3(1), dim3(1024), 0, s, dM, dScratch, (int)r, (int)c,
+                       drank, nullptr, ctx->mc);
+    HIP_TRY(hipGetLastError(), "const_gauss_kernel");
+    int hr = 0;
+    HIP_TRY(hipMemcpyAsync(&hr, drank, sizeof(int), hipMemcpyDeviceToHost, s), "D2H(rank)");
+    HIP_TRY(hipStreamSynchronize(s), "sync");
+    *rank = hr;
+    return PMLG_OK;
+}
+
 }  // namespace pmlg
 
 extern "C" {
