@@ -110,6 +110,16 @@ extern "C" {
 
 /* orientation_t of nmod_poly_mat_forms.h:100-103 */
 typedef enum { COL_UPPER = 0, COL_LOWER = 1, ROW_UPPER = 2, ROW_LOWER = 3 } orientation_t;
+/* poly_mat_form_t of nmod_poly_mat_forms.h:57-66 */
+typedef enum {
+    NONE = 0,
+    REDUCED = 1,
+    WEAK_POPOV = 2,
+    ORD_WEAK_POPOV = 3,
+    POPOV = 4,
+    ECHELON = 5,
+    HERMITE = 6
+} poly_mat_form_t;
 
 void nmod_poly_mat_multiply(nmod_poly_mat_t res, const nmod_poly_mat_t A, const nmod_poly_mat_t B);
 void nmod_poly_mat_mul_geometric(nmod_poly_mat_t res, const nmod_poly_mat_t A,
@@ -135,6 +145,26 @@ void nmod_poly_mat_mbasis(nmod_poly_mat_t appbas, slong *shift, const nmod_poly_
  * `order` of appbas*F] of full rank (rank on the device). */
 int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_poly_mat_t F,
                                        slong order, const slong *shift, orientation_t orient);
+
+/* Minimal kernel bases (nmod_poly_mat_kernel.h:81-105), computed on the
+ * device (pmlg_kernel_basis): the basis occupies the first `nullity` rows
+ * (left kernels) or columns (right kernels) of ker, which keeps its shape;
+ * pivind and shift (in/out, may be NULL for nmod_poly_mat_kernel) hold the
+ * pivot indices and shifted degrees of those rows/columns.  Only the shifted
+ * ordered weak Popov form is produced (form <= ORD_WEAK_POPOV), as in the
+ * reference. */
+slong nmod_poly_mat_kernel(nmod_poly_mat_t ker, slong *pivind, slong *shift,
+                           const nmod_poly_mat_t pmat, poly_mat_form_t form, orientation_t orient);
+slong nmod_poly_mat_kernel_via_approx(nmod_poly_mat_t ker, slong *pivind, slong *shift,
+                                      const nmod_poly_mat_t pmat);
+slong nmod_poly_mat_kernel_zls_approx(nmod_poly_mat_t ker, slong *pivind, slong *shift,
+                                      nmod_poly_mat_t pmat);
+/* verification.c:119-215 (nmod_poly_mat_kernel.h:144-148): dimensions,
+ * nullity = dimension - rank (exact rank by device evaluations), the form
+ * (NONE, WEAK_POPOV or ORD_WEAK_POPOV) and ker*mat == 0 or mat*ker == 0 on the
+ * device.  Generation is not checked, as in the reference. */
+int nmod_poly_mat_is_kernel(const nmod_poly_mat_t ker, const slong *shift,
+                            const nmod_poly_mat_t mat, poly_mat_form_t form, orientation_t orient);
 
 #ifdef __cplusplus
 }

@@ -598,3 +598,306 @@ int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_
     free(R);
     return success;
 }
+
+/* ------------------------------------------------------------------------ */
+/* kernel bases (kernel.c:26-478, verification.c:119-215)                    */
+/* ------------------------------------------------------------------------ */
+
+/* packed copy of F (r x c) with rows and columns exchanged (transpose) and/or
+ * the row order reversed: entry (i, j) of the result is F(ri, cj) where
+ * (ri, cj) = (j, i) when transposed, and the result's row i is taken from
+ * row dim-1-i when mirrored */
+static uint64_t *pack_oriented(const nmod_poly_mat_struct *F, slong len, int transpose, int mirror,
+                               const char *fn)
+{
+    const slong r = transpose ? F->c : F->r, c = transpose ? F->r : F->c;
+    uint64_t *b = xalloc((size_t)(r * c * len), fn);
+    for (slong i = 0; i < r; i++)
+        for (slong j = 0; j < c; j++) {
+            const slong si = mirror ? r - 1 - i : i;
+            const nmod_poly_struct *e =
+                transpose ? nmod_poly_mat_entry(F, j, si) : nmod_poly_mat_entry(F, si, j);
+            uint64_t *d = b + (i * c + j) * len;
+            const slong l = e->length < len ? e->length : len;
+            if (l > 0)
+                memcpy(d, e->coeffs, (size_t)l * sizeof(uint64_t));
+            if (l < len)
+                memset(d + (l > 0 ? l : 0), 0, (size_t)(len - (l > 0 ? l : 0)) * sizeof(uint64_t));
+        }
+    return b;
+}
+
+static void set_entry(nmod_poly_struct *e, const uint64_t *s, slong len)
+{
+    pthread_once(&g_fit_once, resolve_fit);
+    slong l = len;
+    while (l > 0 && s[l - 1] == 0)
+        l--;
+    if (l > 0) {
+        g_fit(e, l);
+        memcpy(e->coeffs, s, (size_t)l * sizeof(uint64_t));
+    }
+    e->length = l;
+}
+
+/* Left kernel of the packed dim x other matrix g on the device; the basis
+ * rows (nul x dim, capacity *cap) are returned in a malloc'd buffer. */
+static uint64_t *kernel_device(const uint64_t *g, slong dim, slong other, slong lenF, uint64_t p,
+                               int64_t *sh, int64_t *piv, int algo, int64_t *nul, int64_t *cap,
+                               const char *fn)
+{
+    pmlg_ctx ctx = ctx_for(p, fn);
+    int64_t amp = 0, mn = dim ? sh[0] : 0, mx = mn;
+    for (slong i = 0; i < dim; i++) {
+        mn = sh[i] < mn ? sh[i] : mn;
+        mx = sh[i] > mx ? sh[i] : mx;
+    }
+    amp = mx - mn;
+    int64_t c = (int64_t)((dim < other ? dim : other) + 1) * (lenF > 0 ? lenF : 1) + amp + 1;
+    int64_t *sh0 = (int64_t *)malloc((size_t)(dim > 0 ? dim : 1) * sizeof(int64_t));
+    if (!sh0)
+        die(fn, "out of host memory");
+    memcpy(sh0, sh, (size_t)dim * sizeof(int64_t));
+    for (;;) {
+        uint64_t *K = xalloc((size_t)(dim * dim * c), fn);
+        int64_t L = 0;
+        memcpy(sh, sh0, (size_t)dim * sizeof(int64_t));
+        pmlg_status st = pmlg_kernel_basis(ctx, K, c, &L, nul, piv, sh, g, dim, other, lenF, algo,
+                                           PMLG_METHOD_AUTO);
+        if (st != PMLG_OK && L > c) {
+            free(K);
+            c = L;
+            continue;
+        }
+        check(st, fn);
+        free(sh0);
+        *cap = c;
+        return K;
+    }
+}
+
+/* kernel.c:26-146 for the four orientations, with the ZLS algorithm */
+static slong kernel_impl(nmod_poly_mat_struct *ker, slong *pivind, slong *shift,
+                         const nmod_poly_mat_struct *pmat, orientation_t orient, int algo,
+                         const char *fn)
+{
+    const int left = orient == ROW_LOWER || orient == ROW_UPPER;
+    const int transpose = !left;
+    const int mirror = orient == ROW_UPPER || orient == COL_LOWER;
+    const slong dim = left ? pmat->r : pmat->c, other = left ? pmat->c : pmat->r;
+    if (left)
+        need_dims(ker->c == dim && ker->r <= dim, fn);
+    else
+        need_dims(ker->r == dim && ker->c <= dim, fn);
+    const slong lenF = pmlg_nmod_poly_mat_max_length(pmat);
+    int64_t *sh = (int64_t *)malloc((size_t)(dim > 0 ? dim : 1) * sizeof(int64_t));
+    int64_t *pv = (int64_t *)malloc((size_t)(dim > 0 ? dim : 1) * sizeof(int64_t));
+    if (!sh || !pv)
+        die(fn, "out of host memory");
+    /* the shift, mirrored with the rows; by default the row (left) or column
+     * (right) degrees of pmat with zero lines taken as 0 */
+    for (slong i = 0; i < dim; i++) {
+        const slong si = mirror ? dim - 1 - i : i;
+        if (shift) {
+            sh[i] = shift[si];
+        } else {
+            slong d = -1;
+            for (slong j = 0; j < other; j++) {
+                const slong e = entry_degree(left ? nmod_poly_mat_entry(pmat, si, j)
+                                                  : nmod_poly_mat_entry(pmat, j, si));
+                d = e > d ? e : d;
+            }
+            sh[i] = d >= 0 ? d : 0;
+        }
+    }
+    if (dim == 0 || other == 0) {
+        /* empty matrix: the identity is a kernel basis (kernel.c:241-249) */
+        for (slong i = 0; i < dim; i++)
+            for (slong j = 0; j < dim; j++) {
+                nmod_poly_struct *e = left ? nmod_poly_mat_entry(ker, i, j)
+                                           : nmod_poly_mat_entry(ker, j, i);
+                const uint64_t one = 1;
+                set_entry(e, &one, i == j ? 1 : 0);
+            }
+        for (slong i = 0; i < dim; i++)
+            if (pivind)
+                pivind[i] = i;
+        free(sh);
+        free(pv);
+        return dim;
+    }
+    uint64_t *g = pack_oriented(pmat, lenF, transpose, mirror, fn);
+    int64_t nul = 0, cap = 0;
+    uint64_t *K = kernel_device(g, dim, other, lenF, pmat->modulus, sh, pv, algo, &nul, &cap, fn);
+    free(g);
+    if (left)
+        need_dims(ker->r >= nul, fn);
+    else
+        need_dims(ker->c >= nul, fn);
+    /* row t of the device basis becomes row (left) or column (right) t of
+     * ker; mirrored orientations reverse the rows, the columns, the shifts
+     * and the pivot indices back (kernel.c:82-105, 108-140) */
+    for (int64_t t = 0; t < nul; t++) {
+        const int64_t st = mirror ? nul - 1 - t : t;
+        for (slong j = 0; j < dim; j++) {
+            const slong sj = mirror ? dim - 1 - j : j;
+            nmod_poly_struct *e = left ? nmod_poly_mat_entry(ker, t, j) : nmod_poly_mat_entry(ker, j, t);
+            set_entry(e, K + (st * dim + sj) * cap, cap);
+        }
+        if (pivind)
+            pivind[t] = mirror ? dim - 1 - pv[st] : pv[st];
+        if (shift)
+            shift[t] = sh[st];
+    }
+    free(K);
+    free(sh);
+    free(pv);
+    return nul;
+}
+
+slong nmod_poly_mat_kernel(nmod_poly_mat_t ker, slong *pivind, slong *shift,
+                           const nmod_poly_mat_t pmat, poly_mat_form_t form, orientation_t orient)
+{
+    static const char *fn = "nmod_poly_mat_kernel";
+    if (form > ORD_WEAK_POPOV)
+        die(fn, "form > ORD_WEAK_POPOV not implemented.");
+    return kernel_impl(ker, pivind, shift, pmat, orient, PMLG_KERNEL_ZLS, fn);
+}
+
+slong nmod_poly_mat_kernel_via_approx(nmod_poly_mat_t ker, slong *pivind, slong *shift,
+                                      const nmod_poly_mat_t pmat)
+{
+    return kernel_impl(ker, pivind, shift, pmat, ROW_LOWER, PMLG_KERNEL_VIA_APPROX,
+                       "nmod_poly_mat_kernel_via_approx");
+}
+
+slong nmod_poly_mat_kernel_zls_approx(nmod_poly_mat_t ker, slong *pivind, slong *shift,
+                                      nmod_poly_mat_t pmat)
+{
+    return kernel_impl(ker, pivind, shift, pmat, ROW_LOWER, PMLG_KERNEL_ZLS,
+                       "nmod_poly_mat_kernel_zls_approx");
+}
+
+/* pivot index of row (left) or column (right) t of M, as
+ * _nmod_poly_vec_pivot_profile with a shift (degrees_pivots_leadingmatrix.c:
+ * 276-365): ROW_LOWER/COL_UPPER take the last entry reaching the shifted
+ * degree, ROW_UPPER/COL_LOWER the first; -1 for a zero vector */
+static slong pivot_of(const nmod_poly_mat_struct *M, slong t, const slong *shift,
+                      orientation_t orient)
+{
+    const int rows = orient == ROW_LOWER || orient == ROW_UPPER;
+    const int last = orient == ROW_LOWER || orient == COL_UPPER;
+    const slong len = rows ? M->c : M->r;
+    slong piv = -1, mx = 0;
+    for (slong j = 0; j < len; j++) {
+        const slong d = entry_degree(rows ? nmod_poly_mat_entry(M, t, j) : nmod_poly_mat_entry(M, j, t));
+        const slong s = shift ? shift[j] : 0;
+        if (d >= 0 && (piv == -1 || (last ? mx <= d + s : mx < d + s))) {
+            piv = j;
+            mx = d + s;
+        }
+    }
+    return piv;
+}
+
+/* nmod_poly_mat_is_form.c: weak Popov (distinct pivots, no zero vector) and
+ * ordered weak Popov (increasing pivots) */
+static int has_form(const nmod_poly_mat_struct *M, const slong *shift, poly_mat_form_t form,
+                    orientation_t orient, const char *fn)
+{
+    if (form == NONE)
+        return 1;
+    if (form != WEAK_POPOV && form != ORD_WEAK_POPOV)
+        die(fn, "only the forms NONE, WEAK_POPOV and ORD_WEAK_POPOV are supported");
+    const slong dim = (orient == ROW_LOWER || orient == ROW_UPPER) ? M->r : M->c;
+    slong *piv = (slong *)malloc((size_t)(dim > 0 ? dim : 1) * sizeof(slong));
+    if (!piv)
+        die(fn, "out of host memory");
+    int ok = 1;
+    for (slong t = 0; t < dim && ok; t++) {
+        piv[t] = pivot_of(M, t, shift, orient);
+        if (piv[t] < 0)
+            ok = 0;
+    }
+    for (slong t = 0; t < dim && ok; t++)
+        for (slong u = (form == ORD_WEAK_POPOV ? t + 1 : 0); u < dim; u++)
+            if (u != t && (form == ORD_WEAK_POPOV ? piv[t] >= piv[u] : piv[t] == piv[u])) {
+                ok = 0;
+                break;
+            }
+    free(piv);
+    return ok;
+}
+
+/* Rank over Z/p(x): a nonzero r x r minor has degree at most r*deg(M), so
+ * it does not vanish at one of the points 0, 1, ..., r*deg(M); each point is
+ * a device evaluation and a device elimination. */
+static slong polmat_rank(const nmod_poly_mat_struct *M, const char *fn)
+{
+    const slong r = M->r, c = M->c;
+    slong d = -1;
+    for (slong i = 0; i < r; i++)
+        for (slong j = 0; j < c; j++) {
+            const slong e = entry_degree(nmod_poly_mat_entry(M, i, j));
+            d = e > d ? e : d;
+        }
+    if (d < 0)
+        return 0;
+    const slong full = r < c ? r : c;
+    const uint64_t p = M->modulus;
+    if ((uint64_t)(full * d) >= p)
+        die(fn, "modulus too small for the exact rank by evaluation");
+    pmlg_ctx ctx = ctx_for(p, fn);
+    const slong len = d + 1;
+    uint64_t *P = pack(M, len, fn);
+    uint64_t *E = xalloc((size_t)(r * c), fn);
+    slong best = 0;
+    for (slong x = 0; x <= full * d && best < full; x++) {
+        int64_t rk = 0;
+        check(pmlg_polmat_eval(ctx, E, P, r, c, len, (uint64_t)x), fn);
+        check(pmlg_const_rank_det(ctx, E, r, c, &rk, NULL), fn);
+        best = rk > best ? rk : best;
+    }
+    free(E);
+    free(P);
+    return best;
+}
+
+/* verification.c:119-215: dimensions, nullity = dim - rank, the requested
+ * form, and ker * mat == 0 (left) or mat * ker == 0 (right), the product on
+ * the device.  As in the reference, generation is not checked. */
+int nmod_poly_mat_is_kernel(const nmod_poly_mat_t ker, const slong *shift,
+                            const nmod_poly_mat_t mat, poly_mat_form_t form, orientation_t orient)
+{
+    static const char *fn = "nmod_poly_mat_is_kernel";
+    const int left = orient == ROW_LOWER || orient == ROW_UPPER;
+    const slong rdim = mat->r, cdim = mat->c;
+    int success = 1;
+    if (left ? ker->c != rdim : ker->r != cdim) {
+        printf(left ? "basis has wrong column dimension\n" : "basis has wrong row dimension\n");
+        success = 0;
+    }
+    const slong rk = polmat_rank(mat, fn);
+    if ((left ? ker->r != rdim - rk : ker->c != cdim - rk)) {
+        printf("number of rows does not equal nullity\n");
+        success = 0;
+    }
+    if (!has_form(ker, shift, form, orient, fn)) {
+        printf("basis does not have the required form\n");
+        success = 0;
+    }
+    if (success && (left ? ker->r : ker->c) > 0) {
+        nmod_poly_mat_t res;
+        pmlg_nmod_poly_mat_init(res, left ? ker->r : rdim, left ? cdim : ker->c, mat->modulus);
+        if (left)
+            mul_impl(res, ker, mat, fn);
+        else
+            mul_impl(res, mat, ker, fn);
+        if (pmlg_nmod_poly_mat_max_length(res) != 0) {
+            printf("not all rows are in the kernel\n");
+            success = 0;
+        }
+        pmlg_nmod_poly_mat_clear(res);
+    }
+    return success;
+}

@@ -11,6 +11,12 @@
         log-uniform in 16..16384, moduli 998244353 / 2^60-3*2^24+1 (50/50),
         inputs generated on the device, one batch plan replayed as a graph.
 
+    python -m pml_amd.perf kernel [--shapes 32x17x1024,...] [--p P] [--algo zls]
+        minimal left kernel bases of random rdim x cdim matrices of degree deg,
+        zero shift, p = 820293421056153991 by default (the prime of the
+        reference's published kernel table, ntl-extras/benchmarks/v01/
+        kernel_zls_app_int.dat); one JSON line per shape.
+
 Each prints one JSON line.  The outputs are checked on the device where the
 check is cheap (the PM-Basis residual P*F = 0 mod x^order; the batch's small
 products against a second configuration).
@@ -152,6 +158,31 @@ def run_batch(a):
     return 0
 
 
+P_KER = 820293421056153991
+
+
+def run_kernel(a):
+    ctx = Context(a.p)
+    for spec in a.shapes.split(","):
+        m, n, d = (int(v) for v in spec.split("x"))
+        rng = np.random.default_rng(m * 10007 + n * 101 + d)
+        F = rng.integers(0, a.p, size=(m, n, d + 1), dtype=np.uint64)
+        shift = np.zeros(m, dtype=np.int64)
+        nul, K, piv, sh = ctx.kernel_basis(F, shift, a.algo)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(a.reps):
+            nul, K, piv, sh = ctx.kernel_basis(F, shift, a.algo)
+        dt = (time.perf_counter() - t0) / a.reps
+        # K * F = 0 on the device
+        zero = not ctx.mul(K, F).any() if nul else True
+        out = {"workload": "kernel basis %dx%d deg %d p=%d shift 0 (%s)" % (m, n, d, a.p, a.algo),
+               "path": "transforms mod p" if ctx.max_log_len >= 12 else "3-prime CRT",
+               "seconds": round(dt, 5), "nullity": int(nul), "basis_length": int(K.shape[2]),
+               "row_degree_sum": int(sh.sum()), "kernel_product_zero": bool(zero), "reps": a.reps}
+        print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m pml_amd.perf")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -167,7 +198,14 @@ def main(argv=None):
     b.add_argument("--reps", type=int, default=3)
     b.add_argument("--check", type=int, default=24)
     b.add_argument("--method", default="auto", choices=["auto", "threshold", "plain", "mfma"])
+    kr = sub.add_parser("kernel")
+    kr.add_argument("--shapes", default="16x5x1024,32x17x1024,64x33x1024,64x49x4096")
+    kr.add_argument("--p", type=int, default=P_KER)
+    kr.add_argument("--algo", default="zls", choices=["zls", "via_approx"])
+    kr.add_argument("--reps", type=int, default=2)
     a = ap.parse_args(argv)
+    if a.cmd == "kernel":
+        return run_kernel(a)
     if a.cmd == "pmbasis":
         return run_pmbasis(a)
     if a.cmd == "mul":

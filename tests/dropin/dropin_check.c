@@ -251,6 +251,128 @@ static void check_basis(int iterative, slong m, slong n, slong lenF, slong order
     nmod_poly_mat_clear(P);
 }
 
+/* shifted degree and pivot (rightmost for ROW_LOWER / COL_UPPER, leftmost
+ * for ROW_UPPER / COL_LOWER) of row or column t, as the reference's
+ * nmod_poly_mat_row_degree / _column_degree and _pivot_index */
+static void line_profile(const nmod_poly_mat_t K, slong t, const slong *shift, orientation_t o,
+                         slong *deg, slong *piv)
+{
+    const int rows = o == ROW_LOWER || o == ROW_UPPER, last = o == ROW_LOWER || o == COL_UPPER;
+    const slong len = rows ? K->c : K->r;
+    slong mx = 0, pv = -1;
+    for (slong j = 0; j < len; j++) {
+        const nmod_poly_struct *e = rows ? nmod_poly_mat_entry(K, t, j) : nmod_poly_mat_entry(K, j, t);
+        if (e->length == 0)
+            continue;
+        const slong v = e->length - 1 + shift[j];
+        if (pv == -1 || (last ? v >= mx : v > mx)) {
+            pv = j;
+            mx = v;
+        }
+    }
+    *deg = mx;
+    *piv = pv;
+}
+
+/* the first `nul` rows (left) or columns (right) of ker as their own matrix */
+static void basis_part(nmod_poly_mat_t out, const nmod_poly_mat_t ker, slong nul, int left)
+{
+    nmod_poly_mat_init(out, left ? nul : ker->r, left ? ker->c : nul, ker->modulus);
+    for (slong i = 0; i < out->r; i++)
+        for (slong j = 0; j < out->c; j++) {
+            const nmod_poly_struct *e = nmod_poly_mat_entry(ker, i, j);
+            nmod_poly_struct *d = nmod_poly_mat_entry(out, i, j);
+            if (e->length) {
+                nmod_poly_fit_length(d, e->length);
+                memcpy(d->coeffs, e->coeffs, (size_t)e->length * sizeof(uint64_t));
+            }
+            d->length = e->length;
+        }
+}
+
+/* Kernel bases through the drop-in, following the reference's kernel test
+ * (t-kernel.c:28-260): for F = G * H of rank r (m x n), every orientation of
+ * nmod_poly_mat_kernel and both left-kernel algorithms give a basis that
+ * nmod_poly_mat_is_kernel accepts, of nullity dim - r, whose shifted degrees
+ * and pivot indices are the returned ones; the two algorithms agree on them
+ * (they depend only on the kernel module and the shift); a perturbed basis
+ * is rejected. */
+static void check_kernel(slong m, slong n, slong len, slong r, uint64_t p)
+{
+    nmod_poly_mat_t G, H, F;
+    nmod_poly_mat_init(G, m, r, p);
+    nmod_poly_mat_init(H, r, n, p);
+    nmod_poly_mat_init(F, m, n, p);
+    rand_mat(G, (len + 1) / 2, 0, 0);
+    rand_mat(H, len - (len + 1) / 2 + 1, 0, 0);
+    nmod_poly_mat_multiply(F, G, H);
+    const orientation_t ors[4] = {ROW_LOWER, ROW_UPPER, COL_LOWER, COL_UPPER};
+    slong piv_first[2][128], deg_first[2][128], nul_first[2] = {-1, -1};
+    for (int a = 0; a < 6; a++) {
+        const orientation_t o = a < 4 ? ors[a] : ROW_LOWER;
+        const int left = o == ROW_LOWER || o == ROW_UPPER;
+        const slong dim = left ? m : n;
+        nmod_poly_mat_t ker;
+        nmod_poly_mat_init(ker, dim, dim, p);
+        slong shift[128], shift0[128], piv[128];
+        for (slong i = 0; i < dim; i++)
+            shift[i] = shift0[i] = (3 * i + (a < 4 ? a : 4)) % 5; /* cases 4, 5: one shift */
+        slong nul;
+        const char *name;
+        if (a < 4) {
+            nul = nmod_poly_mat_kernel(ker, piv, shift, F, ORD_WEAK_POPOV, o);
+            name = "nmod_poly_mat_kernel";
+        } else if (a == 4) {
+            nul = nmod_poly_mat_kernel_via_approx(ker, piv, shift, F);
+            name = "nmod_poly_mat_kernel_via_approx";
+        } else {
+            nul = nmod_poly_mat_kernel_zls_approx(ker, piv, shift, F);
+            name = "nmod_poly_mat_kernel_zls_approx";
+        }
+        if (nul != dim - r)
+            fail("kernel: nullity is not dim - rank");
+        nmod_poly_mat_t kz;
+        basis_part(kz, ker, nul, left);
+        if (!nmod_poly_mat_is_kernel(kz, shift0, F, ORD_WEAK_POPOV, o))
+            fail("kernel: is_kernel rejects the computed basis");
+        for (slong t = 0; t < nul; t++) {
+            slong d, pv;
+            line_profile(kz, t, shift0, o, &d, &pv);
+            if (d != shift[t] || pv != piv[t])
+                fail("kernel: returned degrees / pivots differ from the basis");
+        }
+        if (a >= 4) {
+            const int w = a - 4;
+            nul_first[w] = nul;
+            for (slong t = 0; t < nul; t++) {
+                piv_first[w][t] = piv[t];
+                deg_first[w][t] = shift[t];
+            }
+        }
+        if (nul > 0 && a == 0) {
+            /* a changed low coefficient leaves the kernel */
+            nmod_poly_struct *e = nmod_poly_mat_entry(kz, 0, piv[0]);
+            e->coeffs[0] = (e->coeffs[0] + 1) % p;
+            while (e->length > 0 && e->coeffs[e->length - 1] == 0)
+                e->length--;
+            if (nmod_poly_mat_is_kernel(kz, shift0, F, ORD_WEAK_POPOV, o))
+                fail("kernel: is_kernel accepts a perturbed basis");
+        }
+        printf("ok %s orient %d %ldx%ld rank %ld nullity %ld p=%llu\n", name, (int)o, (long)m,
+               (long)n, (long)r, (long)nul, (unsigned long long)p);
+        nmod_poly_mat_clear(kz);
+        nmod_poly_mat_clear(ker);
+    }
+    if (nul_first[0] != nul_first[1])
+        fail("kernel: the two algorithms disagree on the nullity");
+    for (slong t = 0; t < nul_first[0]; t++)
+        if (piv_first[0][t] != piv_first[1][t] || deg_first[0][t] != deg_first[1][t])
+            fail("kernel: the two algorithms disagree on the pivot profile");
+    nmod_poly_mat_clear(G);
+    nmod_poly_mat_clear(H);
+    nmod_poly_mat_clear(F);
+}
+
 int main(void)
 {
     check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 5, 7, 3, 300, 1, 1, P60);
@@ -276,6 +398,11 @@ int main(void)
     check_basis(1, 4, 2, 40, 40, 2, P30);
     check_basis(1, 7, 3, 25, 25, 0, P60);
     check_basis(0, 5, 2, 90, 90, 1, P_NOFFT);
+    check_kernel(6, 2, 8, 2, P60);
+    check_kernel(7, 4, 5, 2, P30);
+    check_kernel(5, 5, 6, 3, P30);
+    check_kernel(9, 3, 40, 3, P60);
+    check_kernel(4, 6, 3, 1, P_NOFFT);
     printf("all drop-in checks passed\n");
     return 0;
 }
