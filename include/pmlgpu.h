@@ -93,6 +93,25 @@ pmlg_status pmlg_polmat_mulmid_dev(pmlg_ctx ctx, uint64_t *dout, const uint64_t 
                                    int64_t lenA, int64_t lenB, int64_t nlo, int64_t nhi,
                                    int method, void *stream);
 
+/* Products of polynomial matrices held entry by entry in host memory, as
+ * FLINT holds them (nmod_poly_mat_multiply, nmod_poly_mat_mulmid:
+ * nmod_poly_mat_multiply.h:92,122-124).  Entry (i, j) of the r x c matrix X
+ * has lenX[i*c + j] coefficients at X[i*c + j].  The coefficients are packed
+ * straight into pinned staging chunks by several host threads and copied by
+ * DMA while the next chunk is packed; results come back the same way.
+ * out[i*n + j] must have room for nhi - nlo words (lenA + lenB - 1 for the
+ * product, with lenA, lenB the largest entry lengths); lenOut receives each
+ * output entry's length without trailing zeros, and only that many words are
+ * written.  out must not overlap A or B. */
+pmlg_status pmlg_polmat_mul_entries(pmlg_ctx ctx, uint64_t *const *C, int64_t *lenC,
+                                    const uint64_t *const *A, const int64_t *lenA,
+                                    const uint64_t *const *B, const int64_t *lenB, int64_t m,
+                                    int64_t k, int64_t n, int method);
+pmlg_status pmlg_polmat_mulmid_entries(pmlg_ctx ctx, uint64_t *const *out, int64_t *lenOut,
+                                       const uint64_t *const *A, const int64_t *lenA,
+                                       const uint64_t *const *B, const int64_t *lenB, int64_t m,
+                                       int64_t k, int64_t n, int64_t nlo, int64_t nhi, int method);
+
 /* Shift-ordered weak Popov approximant basis of F (m x n, padded length lenF)
  * at order `order` (same order for every column), by divide and conquer on the
  * order (nmod_poly_mat_pmbasis) or iteratively (nmod_poly_mat_mbasis).

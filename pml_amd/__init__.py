@@ -135,6 +135,14 @@ def lib():
                                        c_int]
             L.pmlg_mbasis.restype = st
             L.pmlg_mbasis.argtypes = [vp, _u64p, _i64p, _i64p, _u64p, i64, i64, i64, i64, i64]
+        if hasattr(L, "pmlg_polmat_mul_entries"):
+            pp = ctypes.POINTER(_u64p)
+            L.pmlg_polmat_mul_entries.restype = st
+            L.pmlg_polmat_mul_entries.argtypes = [vp, pp, _i64p, pp, _i64p, pp, _i64p, i64, i64,
+                                                  i64, c_int]
+            L.pmlg_polmat_mulmid_entries.restype = st
+            L.pmlg_polmat_mulmid_entries.argtypes = [vp, pp, _i64p, pp, _i64p, pp, _i64p, i64,
+                                                     i64, i64, i64, i64, c_int]
         if hasattr(L, "pmlg_kernel_basis"):
             L.pmlg_kernel_basis.restype = st
             L.pmlg_kernel_basis.argtypes = [vp, _u64p, i64, _i64p, _i64p, _i64p, _i64p, _u64p, i64,

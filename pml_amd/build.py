@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libpmlgpu.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIP_SOURCES = ["ntt.hip", "pointwise.hip", "api.hip", "crt.hip", "pmbasis.hip", "kernel.hip",
-               "batch.hip", "mlp.hip"]
+               "staging.hip", "batch.hip", "mlp.hip"]
 C_SOURCES = ["dropin.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

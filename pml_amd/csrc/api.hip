@@ -440,6 +440,7 @@ void pmlg_ctx_destroy(pmlg_ctx ctx)
         ctx->tune_ws.release();
         ctx->crt_in.release();
         ctx->crt_out.release();
+        stagers_release(ctx);
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -49,6 +49,19 @@ struct Tables {
     NttTab<uint32_t> t32{};
 };
 
+// Pinned double-buffered staging chunks with their copy stream, one per
+// host thread of the entry-wise transfers (staging.hip).
+constexpr int kMaxStageThreads = 16;
+int stage_threads();     // PMLG_STAGE_THREADS, default 2
+size_t stage_bytes();    // PMLG_STAGE_MIB MiB per chunk, default 2
+struct Stager {
+    void *pin[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+};
+
 struct Prof {
     int stage;
     hipEvent_t a, b;
@@ -107,6 +120,8 @@ struct pmlg_ctx_s {
     bool is_prime = true;
     pmlg_ctx crt[3] = {nullptr, nullptr, nullptr};
     pmlg::DevBuf crt_in, crt_out;
+    // pinned staging for entry-wise host transfers (created on first use)
+    std::vector<pmlg::Stager> stagers;
 };
 
 namespace pmlg {
@@ -145,5 +160,16 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
 // dScratch holding r + 2 words; synchronises s.  ctx->mu held.
 pmlg_status const_rank_dev(pmlg_ctx ctx, uint64_t *dM, uint64_t *dScratch, int64_t r, int64_t c,
                            int64_t *rank, hipStream_t s);
+
+// Entry-wise host transfers through the pinned stagers (ctx->mu held).
+// Upload: device entry e at dst + e*width <- ptr[e] (len[e] words, clipped and
+// zero padded); returns when complete.  Download: ptr[e] <- device entry e
+// of width words, normalised (lenOut[e] = length without trailing zeros),
+// after the work enqueued on `after`.
+pmlg_status stage_upload(pmlg_ctx ctx, uint64_t *dst, const uint64_t *const *ptr,
+                         const int64_t *len, int64_t count, int64_t width);
+pmlg_status stage_download(pmlg_ctx ctx, uint64_t *const *ptr, int64_t *lenOut,
+                           const uint64_t *src, int64_t count, int64_t width, hipStream_t after);
+void stagers_release(pmlg_ctx ctx);
 
 }  // namespace pmlg

@@ -19,6 +19,7 @@
 #define _GNU_SOURCE
 #include <dlfcn.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -295,6 +296,99 @@ static void need_dims(int ok, const char *fn)
 /* products                                                                  */
 /* ------------------------------------------------------------------------ */
 
+/* do the entry structs of X and Y overlap (res aliasing an operand, also
+ * through windows)? */
+static int shares_entries(const nmod_poly_mat_struct *X, const nmod_poly_mat_struct *Y)
+{
+    if (X->r == 0 || X->c == 0 || Y->r == 0 || Y->c == 0)
+        return 0;
+    uintptr_t xl = UINTPTR_MAX, xh = 0, yl = UINTPTR_MAX, yh = 0;
+    for (slong i = 0; i < X->r; i++) {
+        const uintptr_t a = (uintptr_t)nmod_poly_mat_entry(X, i, 0);
+        const uintptr_t b = (uintptr_t)(nmod_poly_mat_entry(X, i, X->c - 1) + 1);
+        xl = a < xl ? a : xl;
+        xh = b > xh ? b : xh;
+    }
+    for (slong i = 0; i < Y->r; i++) {
+        const uintptr_t a = (uintptr_t)nmod_poly_mat_entry(Y, i, 0);
+        const uintptr_t b = (uintptr_t)(nmod_poly_mat_entry(Y, i, Y->c - 1) + 1);
+        yl = a < yl ? a : yl;
+        yh = b > yh ? b : yh;
+    }
+    return xl < yh && yl < xh;
+}
+
+/* coefficient pointers and lengths of M's entries, lengths clipped to clip */
+static void entry_lists(const nmod_poly_mat_struct *M, slong clip, const uint64_t **ptr,
+                        int64_t *len)
+{
+    for (slong i = 0; i < M->r; i++)
+        for (slong j = 0; j < M->c; j++) {
+            const nmod_poly_struct *e = nmod_poly_mat_entry(M, i, j);
+            const slong l = e->length < clip ? e->length : clip;
+            ptr[i * M->c + j] = e->coeffs;
+            len[i * M->c + j] = l > 0 ? l : 0;
+        }
+}
+
+/* res = (A*B mod x^nhi) div x^nlo from the first len1 / len2 coefficients of
+ * the entries of A / B, through the entry-wise transfers of the library: the
+ * operands' coefficient arrays are read in place (no packed host copy) and
+ * the result is written straight into res's entries, which are grown to
+ * nhi - nlo words first.  When res shares entries with A or B the result goes
+ * through a host buffer first, so that growing res cannot move an operand. */
+static void product_entries(nmod_poly_mat_struct *res, const nmod_poly_mat_struct *A, slong len1,
+                            const nmod_poly_mat_struct *B, slong len2, slong nlo, slong nhi,
+                            const char *fn)
+{
+    pthread_once(&g_fit_once, resolve_fit);
+    const slong m = A->r, k = A->c, n = B->c, w = nhi - nlo;
+    const uint64_t **ap = (const uint64_t **)malloc((size_t)(m * k + 1) * sizeof(void *));
+    const uint64_t **bp = (const uint64_t **)malloc((size_t)(k * n + 1) * sizeof(void *));
+    uint64_t **cp = (uint64_t **)malloc((size_t)(m * n + 1) * sizeof(void *));
+    int64_t *al = (int64_t *)malloc((size_t)(m * k + 1) * sizeof(int64_t));
+    int64_t *bl = (int64_t *)malloc((size_t)(k * n + 1) * sizeof(int64_t));
+    int64_t *cl = (int64_t *)malloc((size_t)(m * n + 1) * sizeof(int64_t));
+    if (!ap || !bp || !cp || !al || !bl || !cl)
+        die(fn, "out of host memory");
+    entry_lists(A, len1, ap, al);
+    entry_lists(B, len2, bp, bl);
+    const int alias = shares_entries(res, A) || shares_entries(res, B);
+    uint64_t *buf = NULL;
+    if (alias) {
+        buf = xalloc((size_t)(m * n * w), fn);
+        for (slong e = 0; e < m * n; e++)
+            cp[e] = buf + e * w;
+    } else {
+        for (slong i = 0; i < m; i++)
+            for (slong j = 0; j < n; j++) {
+                nmod_poly_struct *e = nmod_poly_mat_entry(res, i, j);
+                g_fit(e, w);
+                cp[i * n + j] = e->coeffs;
+            }
+    }
+    check(pmlg_polmat_mulmid_entries(ctx_for(A->modulus, fn), cp, cl, ap, al, bp, bl, m, k, n,
+                                     nlo, nhi, PMLG_METHOD_AUTO),
+          fn);
+    for (slong i = 0; i < m; i++)
+        for (slong j = 0; j < n; j++) {
+            nmod_poly_struct *e = nmod_poly_mat_entry(res, i, j);
+            const slong l = cl[i * n + j];
+            if (alias && l > 0) {
+                g_fit(e, l);
+                memcpy(e->coeffs, cp[i * n + j], (size_t)l * sizeof(uint64_t));
+            }
+            e->length = l;
+        }
+    free(buf);
+    free(ap);
+    free(bp);
+    free(cp);
+    free(al);
+    free(bl);
+    free(cl);
+}
+
 /* nmod_poly_mat_multiply (mul.c:19-97): the product is unique, so the
  * reference's choice among its evaluation back-ends has no counterpart here;
  * the device picks the pointwise kernel (int8-limb MFMA or plain) itself. */
@@ -308,16 +402,7 @@ static void mul_impl(nmod_poly_mat_struct *res, const nmod_poly_mat_struct *A,
         pmlg_nmod_poly_mat_zero(res);
         return;
     }
-    const slong m = A->r, k = A->c, n = B->c, lenC = lenA + lenB - 1;
-    uint64_t *a = pack(A, lenA, fn);
-    uint64_t *b = pack(B, lenB, fn);
-    uint64_t *c = xalloc((size_t)(m * n * lenC), fn);
-    check(pmlg_polmat_mul(ctx_for(A->modulus, fn), c, a, b, m, k, n, lenA, lenB, PMLG_METHOD_AUTO),
-          fn);
-    free(a);
-    free(b);
-    unpack(res, c, lenC); /* A and B are packed already: res may alias them */
-    free(c);
+    product_entries(res, A, lenA, B, lenB, 0, lenA + lenB - 1, fn);
 }
 
 void nmod_poly_mat_multiply(nmod_poly_mat_t res, const nmod_poly_mat_t A, const nmod_poly_mat_t B)
@@ -363,17 +448,7 @@ static void mulmid_impl(nmod_poly_mat_struct *res, const nmod_poly_mat_struct *A
         pmlg_nmod_poly_mat_zero(res);
         return;
     }
-    const slong m = A->r, k = A->c, n = B->c, lenR = nhi - nlo;
-    uint64_t *a = pack(A, len1, fn);
-    uint64_t *b = pack(B, len2, fn);
-    uint64_t *r = xalloc((size_t)(m * n * lenR), fn);
-    check(pmlg_polmat_mulmid(ctx_for(A->modulus, fn), r, a, b, m, k, n, len1, len2, nlo, nhi,
-                             PMLG_METHOD_AUTO),
-          fn);
-    free(a);
-    free(b);
-    unpack(res, r, lenR);
-    free(r);
+    product_entries(res, A, len1, B, len2, nlo, nhi, fn);
 }
 
 void _nmod_poly_mat_mulmid(nmod_poly_mat_t res, const nmod_poly_mat_t A, slong len1,
