@@ -7,13 +7,15 @@ A step is one batch of B independent products C_i = A_i * B_i run through a
 batch plan (one HIP-graph replay: forward transforms of every operand at 8192
 points, 8192 pointwise 32x32x32 modular matrix products per product, inverse
 transforms), each product's configuration picked by the learned dispatch
-(the packaged network trained on recorded timings); inputs and outputs stay
-resident in HBM.  `value` is products per second of that batched+learned
-form.  Beside it: the plain GPU form (one product per call, plain per-lane
-pointwise kernel), the single-call MFMA form, and the batched form under the
-threshold rule.  Prints ONE JSON line.  The CPU baseline is the oracle's own
-NTT product (a single-threaded C port of the same algorithm) timed on this
-host on a bounded sample.
+(the packaged network trained on recorded timings, refined by one
+measurement among the configurations it predicts within 5% of its best);
+inputs and outputs stay resident in HBM.  `value` is products per second of
+that batched+learned form.  Beside it: the plain GPU form (one product per
+call, plain per-lane pointwise kernel), the single-call MFMA form, the batched
+form under the threshold rule, and (when the refinement changed the pick) the
+batched form with the network's own pick.  Prints ONE JSON line.  The CPU
+baseline is the oracle's own NTT product (a single-threaded C port of the
+same algorithm) timed on this host on a bounded sample.
 """
 import argparse
 import json
@@ -40,7 +42,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=4, help="products per step")
+    ap.add_argument("--batch", type=int, default=16, help="products per step")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -132,6 +134,11 @@ def main():
     Bs = [rng.integers(0, P60, size=(K, N_, LEN), dtype=np.uint64) for _ in range(Bn)]
     ctx = pml_amd.Context(P60)
     model = learn.install_default()
+    # the network's own pick, and the pick after the measured refinement among
+    # the configurations it predicts within 5% of its best
+    pml_amd.set_refine(False)
+    model_choice = pml_amd.choose_config(ctx, M, K, N_, LEN, LEN, "auto")
+    pml_amd.set_refine(True)
     chosen = pml_amd.choose_config(ctx, M, K, N_, LEN, LEN, "auto")
 
     plan = pml_amd.BatchPlan([(ctx, As[i], Bs[i]) for i in range(Bn)], method="auto", graph=True)
@@ -151,6 +158,10 @@ def main():
         with pml_amd.BatchPlan([(ctx, As[i], Bs[i]) for i in range(Bn)], method="threshold",
                                graph=True) as tplan:
             forms["batched_threshold"] = n2 * Bn / time_plan(tplan, n2, 2)
+        if model_choice != chosen:
+            with pml_amd.BatchPlan([(ctx, As[i], Bs[i]) for i in range(Bn)], config=model_choice,
+                                   graph=True) as mplan:
+                forms["batched_learned_unrefined"] = n2 * Bn / time_plan(mplan, n2, 2)
 
     if args.copy_mib:
         nb = args.copy_mib << 20
@@ -167,17 +178,20 @@ def main():
     dom = max(range(3), key=lambda i: ms[i])
     algo = stage_bytes(Bn)
     achieved = algo[names[dom]] / (ms[dom] * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            tr = json.load(f)
-        key = "%s_batch%d" % (names[dom], Bn)
-        if key in tr:
-            traffic = tr[key]["hbm_bytes_per_launch"]
     # kernels of the chosen configuration (transform radices, pointwise kernel)
     fe, ie, kind, pbs = {1: (5, 4, "plain", 0), 2: (5, 4, "mfma", 0), 3: (4, 4, "plain", 0),
                          4: (4, 4, "mfma", 0), 5: (5, 5, "mfma", 0), 6: (5, 4, "mfma", 1),
                          7: (5, 5, "plain", 0)}.get(chosen, (5, 4, "mfma", 0))
+    # measured HBM traffic of the same kernel variant at this batch size
+    variant = {"forward": "loge%d" % fe, "inverse": "loge%d" % ie,
+               "pointwise": "mfma" if kind == "mfma" and pbs == 0 else "other"}
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        key = "%s_batch%d_%s" % (names[dom], Bn, variant[names[dom]])
+        if key in tr:
+            traffic = tr[key]["hbm_bytes_per_launch"]
     kernels = {"forward": "ntt_fwd_segs_reg<u64,u64,loose,L=13,LOGE=%d>" % fe,
                "pointwise": ("pw_mfma_segs<u64,8,%d>" % (4 >> pbs)) if kind == "mfma"
                else "pw_plain_segs<u64>",
@@ -225,7 +239,8 @@ def main():
         "data": "synthetic (seeded uniform residues mod p)",
         "config": {"workload": "polmat_mul 32x32*32x32, length 4096 each, p=2^60-3*2^24+1, "
                                "NTT length 8192, 8192 pointwise 32x32x32 products per product",
-                   "batch": Bn, "dispatch": "learned", "config_chosen": chosen,
+                   "batch": Bn, "dispatch": "learned+refined", "config_chosen": chosen,
+                   "config_model": model_choice,
                    "parity_entry00": ok00, "parity_eval_identity_all_entries": okr},
         "forms_products_per_s": {k: round(v, 3) for k, v in forms.items()},
         "vs_plain_gpu": round(value / forms["plain_single"], 3) if "plain_single" in forms else None,

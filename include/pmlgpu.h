@@ -21,6 +21,8 @@
  *                           _nmod_poly_mat_mulmid     nmod_poly_mat_multiply.h:128-131
  *   pmlg_pmbasis         <- nmod_poly_mat_pmbasis     nmod_poly_mat_approximant.h:282-285
  *   pmlg_mbasis          <- nmod_poly_mat_mbasis      nmod_poly_mat_approximant.h:321-324
+ *   pmlg_kernel_basis    <- nmod_poly_mat_kernel_zls_approx / _via_approx
+ *                                                     nmod_poly_mat_kernel.h:91-105
  *   pmlg_polmat_mul_batch: no single reference counterpart; it runs many
  *                          independent nmod_poly_mat_multiply calls at once.
  *
@@ -228,7 +230,10 @@ int pmlg_choose_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t le
 /* Autotuner: times every valid configuration on device-generated inputs of
  * this shape (one warm-up and `reps` timed runs each; ms_per_config[c] = -1
  * for a configuration not timed), records the fastest for the shape class in
- * the context, and returns it in *best_config. */
+ * the context, and returns it in *best_config.  Each configuration runs a
+ * plan of up to 16 identical products (enough for about 2^27 words of
+ * evaluations, as in the shared launches of a real plan); the times are per
+ * product. */
 pmlg_status pmlg_autotune(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA,
                           int64_t lenB, int reps, double *ms_per_config, int *best_config);
 
@@ -266,6 +271,12 @@ void pmlg_mlp_features(int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t le
                        float *x);
 /* Model consulted by PMLG_METHOD_AUTO (NULL: none; the caller keeps it alive). */
 pmlg_status pmlg_dispatch_set_model(pmlg_mlp mlp);
+/* Measured refinement of the model's choice (on by default; PMLG_REFINE=0 in
+ * the environment or this call turns it off): when configurations other
+ * than the model's best are predicted within 5% of it, those candidates are
+ * timed once per shape class, on replicated products as in pmlg_autotune,
+ * and the fastest is used for every later product of the class. */
+void pmlg_dispatch_set_refine(int on);
 
 /* Per-stage device timing (HIP events around each launch).  Stages:
  * 0 forward NTT, 1 pointwise product, 2 inverse NTT (+ merge). */

@@ -135,6 +135,8 @@ def lib():
                                        c_int]
             L.pmlg_mbasis.restype = st
             L.pmlg_mbasis.argtypes = [vp, _u64p, _i64p, _i64p, _u64p, i64, i64, i64, i64, i64]
+        L.pmlg_dispatch_set_refine.restype = None
+        L.pmlg_dispatch_set_refine.argtypes = [c_int]
         if hasattr(L, "pmlg_polmat_mul_entries"):
             pp = ctypes.POINTER(_u64p)
             L.pmlg_polmat_mul_entries.restype = st
@@ -349,6 +351,13 @@ class Context:
 # ---------------------------------------------------------------------------
 def num_configs():
     return lib().pmlg_num_configs()
+
+
+def set_refine(on):
+    """Turn the measured refinement of the learned choice on or off
+    (pmlg_dispatch_set_refine): with it, configurations predicted within 5%
+    of the model's best are timed once per shape class."""
+    lib().pmlg_dispatch_set_refine(1 if on else 0)
 
 
 class BatchPlan:

@@ -18,6 +18,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <vector>
@@ -188,9 +191,36 @@ int choose_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, i
         return 2;
     if (method == PMLG_METHOD_THRESHOLD)
         return threshold_config(ctx, m, k, n, lenA, lenB);
-    int c = learned_config(ctx, m, k, n, lenA, lenB);
-    if (c >= 0 && config_valid(ctx, m, k, n, lenA, lenB, c))
+    float pred[kNumConfigs];
+    int c = learned_config(ctx, m, k, n, lenA, lenB, pred);
+    if (c >= 0 && config_valid(ctx, m, k, n, lenA, lenB, c)) {
+        // candidates the model cannot separate: predicted within kRefineMargin of its best
+        unsigned mask = 0;
+        int ncand = 0;
+        const float bound = pred[c] + (float)std::log1p(kRefineMargin);
+        for (int cc = 0; cc < kNumConfigs; ++cc)
+            if (pred[cc] <= bound) {
+                mask |= 1u << cc;
+                ++ncand;
+            }
+        if (ncand <= 1 || !refine_enabled())
+            return c;
+        const uint64_t key = tune_key(ctx, m, k, n, lenA, lenB);
+        {
+            std::lock_guard<std::mutex> lk(ctx->tune_mu);
+            auto it = ctx->refined.find(key);
+            if (it != ctx->refined.end() && config_valid(ctx, m, k, n, lenA, lenB, it->second))
+                return it->second;
+        }
+        double ms[kNumConfigs];
+        int best = -1;
+        if (time_configs(ctx, m, k, n, lenA, lenB, 1, ms, &best, mask) == PMLG_OK && best >= 0) {
+            std::lock_guard<std::mutex> lk(ctx->tune_mu);
+            ctx->refined[key] = best;
+            return best;
+        }
         return c;
+    }
     {
         std::lock_guard<std::mutex> lk(ctx->tune_mu);
         auto it = ctx->tuned.find(tune_key(ctx, m, k, n, lenA, lenB));
@@ -699,22 +729,41 @@ pmlg_status run_plan(pmlg_batch_plan plan, hipStream_t s)
 // ---------------------------------------------------------------------------
 namespace pmlg {
 
+static std::atomic<int> g_refine{-1};
+
+bool refine_enabled()
+{
+    int v = g_refine.load();
+    if (v < 0) {
+        const char *e = getenv("PMLG_REFINE");
+        v = (e && e[0] == '0') ? 0 : 1;
+        g_refine.store(v);
+    }
+    return v != 0;
+}
+
 pmlg_status time_configs(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
-                         int reps, double *ms, int *best)
+                         int reps, double *ms, int *best, unsigned mask)
 {
     if (m <= 0 || k <= 0 || n <= 0 || lenA <= 0 || lenB <= 0 || reps <= 0 || !ms)
         return fail(PMLG_ERR_ARG, "autotune: bad shape");
     DeviceGuard g(ctx->device);
     hipStream_t s = ctx->stream;
     const int64_t lenC = lenA + lenB - 1;
+    const int64_t N = (int64_t)1 << ceil_log2(lenC);
+    // A plan runs each configuration's products inside launches shared with
+    // many others, so a product is timed among `copies` identical ones (same
+    // inputs, own outputs and evaluations) in one plan: enough copies that the
+    // launches hold about 2^27 evaluation words, at most kTuneCopies.
+    const double eval_words = (double)(m * k + k * n + m * n) * (double)N;
+    const int copies = (int)std::min<double>(kTuneCopies, std::max(1.0, std::ceil(134217728.0 / eval_words)));
     const size_t in_words = (size_t)(m * k * lenA + k * n * lenB);
     const size_t out_words = (size_t)(m * n * lenC);
     // evaluation arena bound: 4 buffers of the largest entry count at the transform length
-    const int64_t N = (int64_t)1 << ceil_log2(lenC);
     const int64_t ent = std::max(m * k, std::max(k * n, m * n));
-    const size_t ws_bytes = (size_t)(4 * ent * N) * 8 + 4096 * 4 + (1 << 20);
+    const size_t ws_bytes = (size_t)(4 * ent * N) * 8 * copies + 4096 * 4 * copies + (1 << 20);
     std::lock_guard<std::mutex> lk(ctx->tune_mu);
-    HIP_TRY(ctx->tune_io.ensure((in_words + out_words) * 8 + 256), "hipMalloc(autotune io)");
+    HIP_TRY(ctx->tune_io.ensure((in_words + out_words * copies) * 8 + 256), "hipMalloc(autotune io)");
     HIP_TRY(ctx->tune_ws.ensure(ws_bytes), "hipMalloc(autotune workspace)");
     uint64_t *A = static_cast<uint64_t *>(ctx->tune_io.ptr);
     uint64_t *B = A + m * k * lenA;
@@ -730,26 +779,30 @@ pmlg_status time_configs(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t 
     pmlg_status result = PMLG_OK;
     for (int c = 0; c < kNumConfigs && result == PMLG_OK; ++c) {
         ms[c] = -1.0;
-        if (!config_valid(ctx, m, k, n, lenA, lenB, c))
+        if (!(mask & (1u << c)) || !config_valid(ctx, m, k, n, lenA, lenB, c))
             continue;
         // direct convolution is only timed where it can compete
         if (kConfigs[c].school &&
             (double)m * n * k * lenC * std::min(lenA, lenB) > kSchoolTimeCap)
             continue;
-        pmlg_mul_desc d{};
-        d.ctx = ctx;
-        d.A = A;
-        d.B = B;
-        d.C = C;
-        d.m = m;
-        d.k = k;
-        d.n = n;
-        d.lenA = lenA;
-        d.lenB = lenB;
-        d.method = PMLG_METHOD_AUTO;
-        d.config = c;
+        std::vector<pmlg_mul_desc> d((size_t)copies);
+        for (int i = 0; i < copies; ++i) {
+            d[(size_t)i] = pmlg_mul_desc{};
+            d[(size_t)i].ctx = ctx;
+            d[(size_t)i].A = A;
+            d[(size_t)i].B = B;
+            d[(size_t)i].C = C + (size_t)i * out_words;
+            d[(size_t)i].m = m;
+            d[(size_t)i].k = k;
+            d[(size_t)i].n = n;
+            d[(size_t)i].lenA = lenA;
+            d[(size_t)i].lenB = lenB;
+            d[(size_t)i].method = PMLG_METHOD_AUTO;
+            d[(size_t)i].config = c;
+        }
         pmlg_batch_plan plan = nullptr;
-        pmlg_status st = build_plan(&plan, &d, 1, 0, ctx->tune_ws.ptr, ctx->tune_ws.bytes, nullptr);
+        pmlg_status st = build_plan(&plan, d.data(), copies, 0, ctx->tune_ws.ptr, ctx->tune_ws.bytes,
+                                    nullptr);
         if (st != PMLG_OK) {
             result = st;
             break;
@@ -764,7 +817,7 @@ pmlg_status time_configs(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t 
             (void)hipEventSynchronize(e1);
             float t = 0;
             (void)hipEventElapsedTime(&t, e0, e1);
-            ms[c] = (double)t / reps;
+            ms[c] = (double)t / reps / copies;
             if (best_c < 0 || ms[c] < best_ms) {
                 best_ms = ms[c];
                 best_c = c;
@@ -781,7 +834,7 @@ pmlg_status time_configs(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t 
         return result;
     if (best)
         *best = best_c;
-    if (best_c >= 0)
+    if (best_c >= 0 && mask == kAllConfigs)
         ctx->tuned[tune_key(ctx, m, k, n, lenA, lenB)] = best_c;
     return PMLG_OK;
 }
@@ -922,6 +975,11 @@ pmlg_status pmlg_fill_random(pmlg_ctx ctx, uint64_t *dst, int64_t n, uint64_t se
     hipLaunchKernelGGL(fill_random_kernel, dim3(grid_for(n)), dim3(256), 0, s, dst, n, ctx->p, seed);
     HIP_TRY(hipGetLastError(), "fill_random");
     return PMLG_OK;
+}
+
+void pmlg_dispatch_set_refine(int on)
+{
+    g_refine.store(on ? 1 : 0);
 }
 
 int pmlg_num_configs(void)

@@ -114,6 +114,8 @@ struct pmlg_ctx_s {
     // autotuner: best configuration per shape class, and its timing buffers
     std::mutex tune_mu;
     std::map<uint64_t, int> tuned;
+    // measured refinement of the learned choice, per shape class
+    std::map<uint64_t, int> refined;
     pmlg::DevBuf tune_io, tune_ws;
     // moduli without the roots a product needs: products modulo three FFT
     // primes (contexts created on first use) recombined by CRT

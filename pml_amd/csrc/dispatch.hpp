@@ -21,19 +21,30 @@ extern const ProdConfig kConfigs[kNumConfigs];
 // timed by the autotuner while m n k lenC min(lenA, lenB) <= kSchoolTimeCap
 constexpr int64_t kSchoolMaxLen = 256;
 constexpr int64_t kSchoolThreshold = 8;
+// most identical products the autotuner times together per configuration
+constexpr int kTuneCopies = 16;
 constexpr double kSchoolTimeCap = 8.6e9;
 
 bool config_valid(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB, int c);
 int threshold_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB);
 int choose_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
                   int method);
-// configuration picked by the learned cost model, or -1 when none is loaded
-int learned_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB);
+// configuration picked by the learned cost model, or -1 when none is loaded;
+// pred (if set) receives the predicted log(ms) of every configuration
+int learned_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                   float *pred = nullptr);
+// The model's top candidates whose predicted times lie within kRefineMargin
+// of its best are timed once per shape class (measured refinement).
+constexpr double kRefineMargin = 0.05;
+bool refine_enabled();
 
 pmlg_status build_plan(pmlg_batch_plan *out, const pmlg_mul_desc *descs, int64_t count, int flags,
                        void *ws, size_t ws_bytes, size_t *need_bytes);
 pmlg_status run_plan(pmlg_batch_plan plan, hipStream_t s);
+// Times the configurations in `mask` (bit c: configuration c) for one shape;
+// with the full mask the winner also goes to the autotuner cache.
+constexpr unsigned kAllConfigs = (1u << kNumConfigs) - 1;
 pmlg_status time_configs(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
-                         int reps, double *ms, int *best);
+                         int reps, double *ms, int *best, unsigned mask = kAllConfigs);
 
 }  // namespace pmlg

@@ -302,7 +302,8 @@ void mlp_features(int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB, u
     x[4] = (float)(64 - __builtin_clzll(p)) / 64.0f;
 }
 
-int learned_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB)
+int learned_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                   float *pred)
 {
     std::lock_guard<std::mutex> lk(g_model_mu);
     if (!g_model)
@@ -312,9 +313,13 @@ int learned_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, 
     if (pmlg_mlp_forward(g_model, x, 1, y) != PMLG_OK)
         return -1;
     int best = -1;
-    for (int c = 0; c < PMLG_MLP_OUT && c < kNumConfigs; ++c)
-        if (config_valid(ctx, m, k, n, lenA, lenB, c) && (best < 0 || y[c] < y[best]))
+    for (int c = 0; c < PMLG_MLP_OUT && c < kNumConfigs; ++c) {
+        const bool ok = config_valid(ctx, m, k, n, lenA, lenB, c);
+        if (pred)
+            pred[c] = ok ? y[c] : INFINITY;
+        if (ok && (best < 0 || y[c] < y[best]))
             best = c;
+    }
     return best;
 }
 

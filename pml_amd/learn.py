@@ -6,7 +6,9 @@
 `collect` draws shapes of the uneven-batch workload (m, k, n uniform in
 2..64, each length log-uniform in 16..16384, the modulus a 30-bit or a 60-bit
 FFT prime with equal probability) and records, for each, the device time of
-every valid configuration (pmlg_autotune: one warm-up and `reps` timed runs).
+every valid configuration (pmlg_autotune: one warm-up and `reps` timed runs,
+each product timed among up to 16 identical ones, as products share launches
+in a real plan).
 `train` fits the fp32 network on the device (pmlg_mlp_train: the library's own
 forward, backward and Adam kernels) on log(ms) of the first shapes and
 evaluates on the held-out rest: total recorded time of the configurations the

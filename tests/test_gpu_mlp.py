@@ -86,6 +86,7 @@ def test_learned_dispatch_picks_argmin_and_stays_exact(learn):
     ctxs = {P60: pml_amd.Context(P60), P30: pml_amd.Context(P30)}
     try:
         mlp.install()
+        pml_amd.set_refine(False)   # the model's own argmin
         shapes = [(P60, 5, 7, 3, 100, 90), (P30, 9, 2, 4, 300, 17), (P60, 2, 40, 2, 33, 600),
                   (P30, 12, 12, 12, 64, 64), (P60, 3, 3, 3, 1500, 1200)]
         items, seen = [], set()
@@ -105,10 +106,41 @@ def test_learned_dispatch_picks_argmin_and_stays_exact(learn):
         for (p, m, k, n, la, lb) in shapes:
             assert pml_amd.choose_config(ctxs[p], m, k, n, la, lb, "threshold") in (0, 1, 2)
     finally:
+        pml_amd.set_refine(True)
         learn.Mlp.uninstall()
         mlp.close()
         for c in ctxs.values():
             c.close()
+
+
+def test_refinement_picks_among_close_candidates(learn):
+    """With refinement on, a shape whose predicted times lie within 5% of
+    the model's best gets one of those candidates, the same one on every later
+    call (cached per shape class), and exact products."""
+    import math
+    import pml_amd
+    mlp = learn.Mlp.load(learn.DEFAULT_MODEL)
+    ctx = pml_amd.Context(P60)
+    try:
+        mlp.install()
+        pml_amd.set_refine(True)
+        shapes = [(32, 32, 32, 4096, 4096), (24, 17, 30, 900, 1300), (7, 40, 9, 3000, 200)]
+        items = []
+        for (m, k, n, la, lb) in shapes:
+            pred = mlp.forward(learn.features(m, k, n, la, lb, P60)[None, :])[0]
+            valid = [c for c in range(8) if pml_amd.config_valid(ctx, m, k, n, la, lb, c)]
+            best = min(pred[c] for c in valid)
+            cand = {c for c in valid if pred[c] <= best + math.log1p(0.05) + 1e-6}
+            got = pml_amd.choose_config(ctx, m, k, n, la, lb, "auto")
+            assert got in cand, (got, cand)
+            assert pml_amd.choose_config(ctx, m, k, n, la, lb, "auto") == got
+            items.append((ctx, O.rand_uniform((m, k, la), P60, la), O.rand_uniform((k, n, lb), P60, lb)))
+        for (c_, A, B), C in zip(items, pml_amd.mul_batch(items, method="auto")):
+            assert np.array_equal(C, O.polmat_mul(A, B, P60, method="ntt"))
+    finally:
+        learn.Mlp.uninstall()
+        mlp.close()
+        ctx.close()
 
 
 def test_learned_dispatch_gains_over_threshold_on_unseen_shapes(learn):
