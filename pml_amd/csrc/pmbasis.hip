@@ -26,6 +26,7 @@
 // stays on the device: a leaf is a fixed sequence of launches.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include <climits>
 
@@ -38,6 +39,7 @@ namespace {
 
 constexpr int kStepThreads = 1024;
 constexpr int kMaxDim = 128;  // m, n limits of the device M-Basis step
+constexpr int kPanel = 4;     // Gauss-Jordan columns composed per pair of barriers
 
 // control words of one leaf
 enum Ctl {
@@ -92,20 +94,50 @@ __device__ uint64_t invmod_g(uint64_t a, const ModConst &mc)
     return r;
 }
 
-// Arithmetic of the step kernel.  Values in LDS are kept in the policy's
-// representation: plain residues (Plain), or Montgomery residues x 2^32 mod p
-// for p < 2^30 (Mont32: one 32x32 -> 64 product, one low product and one
-// multiply-add per reduction instead of a 64-bit Barrett step).  Zero maps to
-// zero either way, so pivot tests read the representation directly.
-struct PlainArith {
-    using V = uint64_t;  // word held in LDS
-    ModConst mc;
-    __device__ explicit PlainArith(const ModConst &c) : mc(c) {}
-    __device__ V to(uint64_t x) const { return x; }
-    __device__ uint64_t from(V x) const { return x; }
-    __device__ V mul(V a, V b) const { return mulmod_g(a, b, mc); }
-    __device__ V sub(V a, V b) const { return submod_g(a, b, mc.p); }
-    __device__ V inv(V a) const { return invmod_g(a, mc); }
+// Arithmetic of the step kernel (p an odd prime below 2^62).  Values in LDS
+// are Montgomery residues: x 2^32 mod p for p < 2^30 (Mont32: one 32x32 -> 64
+// product, one low product and one multiply-add per reduction), x 2^64 mod p
+// otherwise (Mont64: a 64x64 -> 128 product, one low and one high 64-bit
+// product, against two Barrett steps and a Shoup step for a plain 128-bit
+// reduction).  Zero maps to zero, so pivot tests read the representation
+// directly.
+struct Mont64Arith {
+    using V = uint64_t;
+    uint64_t p, pinv, r1, r2;  // pinv = -p^-1 mod 2^64, r1 = 2^64 mod p, r2 = 2^128 mod p
+    __device__ explicit Mont64Arith(const ModConst &c)
+        : p(c.p), pinv(c.pinvneg), r1(c.r64), r2(mulmod_g(c.r64, c.r64, c)) {}
+    __device__ V redc(uint64_t hi, uint64_t lo) const
+    {
+        // (hi 2^64 + lo + mm p) / 2^64 with mm = lo pinv mod 2^64: the low
+        // words sum to 0 when lo = 0 and to 2^64 otherwise; for inputs below
+        // p^2 the quotient is below 2p < 2^63
+        const uint64_t mm = lo * pinv;
+        const uint64_t u = hi + mulhi64(mm, p) + (lo != 0 ? 1 : 0);
+        return u >= p ? u - p : u;
+    }
+    __device__ V mul(V a, V b) const { return redc(mulhi64(a, b), a * b); }
+    __device__ V sub(V a, V b) const { return a >= b ? a - b : a + (p - b); }
+    __device__ V add(V a, V b) const
+    {
+        const uint64_t t = a + b;  // < 2p < 2^63
+        return t >= p ? t - p : t;
+    }
+    __device__ V one() const { return r1; }
+    __device__ V to(uint64_t x) const { return mul(x, r2); }
+    __device__ uint64_t from(V x) const { return redc(0, x); }
+    __device__ V inv(V a) const
+    {
+        // Fermat a^(p-2), in the Montgomery domain (r1 is 1 there)
+        uint64_t e = p - 2;
+        V r = r1, b = a;
+        while (e) {
+            if (e & 1)
+                r = mul(r, b);
+            b = mul(b, b);
+            e >>= 1;
+        }
+        return r;
+    }
 };
 
 struct Mont32Arith {
@@ -129,6 +161,13 @@ struct Mont32Arith {
         const uint32_t d = a - b, e = d + p;
         return e < d ? e : d;
     }
+    __device__ V add(V a, V b) const
+    {
+        // a + b < 2p < 2^31: min(s, s - p) as unsigned words
+        const uint32_t t = a + b, u = t - p;
+        return u < t ? u : t;
+    }
+    __device__ V one() const { return r1; }
     __device__ V to(uint64_t x) const { return mul((V)x, r2); }
     __device__ uint64_t from(V x) const { return redc(x); }
     __device__ V inv(V a) const
@@ -154,6 +193,13 @@ struct Mont32Arith {
 // column); for a pivot at (pr, c) every other row r becomes
 // pv * row_r - At[r][c] * row_pr with pv = At[pr][c].  The k-th row of the
 // reduced row echelon form is row prow[k] divided by its pivot.
+//
+// The columns are taken kPanel at a time.  One wave runs the pivot steps of
+// the panel on the panel's columns only and composes them: every row ends as
+// C[r][0] * (its own row at the panel's start) + sum_j C[r][j+1] * (the pivot
+// row of panel column j at the panel's start), the same exact residues as the
+// one-column-at-a-time sequence.  All threads then apply that rank-(1 + pivots) combination to the
+// whole matrix: two barriers per panel instead of one per column.
 template <typename Arith>
 __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, int ord)
 {
@@ -168,17 +214,18 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // 64-bit words first (sh), then the V arrays, then the int arrays
     int64_t *sh = reinterpret_cast<int64_t *>(smem);             // kMaxDim
     V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // npad x ms
-    V *fcol = At + npad * ms;                                    // 2 x kMaxDim: column snapshots
-    V *inv = fcol + 2 * kMaxDim;                                 // kMaxDim
-    int *perm = reinterpret_cast<int *>(inv + kMaxDim);          // kMaxDim
+    V *inv = At + npad * ms;                                     // kMaxDim
+    V *Ck = inv + kMaxDim;                                       // kMaxDim x (kPanel + 1)
+    V *Rp = Ck + kMaxDim * (kPanel + 1);                         // kPanel x kMaxDim
+    int *perm = reinterpret_cast<int *>(Rp + kPanel * kMaxDim);  // kMaxDim
     int *pivcol = perm + kMaxDim;                                // kMaxDim
     int *plist = pivcol + kMaxDim;                               // kMaxDim
     int *prow = plist + kMaxDim;                                 // kMaxDim
     int *ispiv = prow + kMaxDim;                                 // kMaxDim (rows of At)
     int *colpiv = ispiv + kMaxDim;                               // kMaxDim (columns of At)
     int *pos = colpiv + kMaxDim;                                 // kMaxDim
-    int *cflag = pos + kMaxDim;                                  // 2 x kMaxDim: pivot candidates
     __shared__ int s_state[2];
+    __shared__ int s_nb, s_mask;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint64_t p = a.mc.p;
@@ -231,64 +278,141 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         for (int k = 0; k < rpt; ++k) {
             const int r = r0 + k * rstep;
             const V v = r < n ? ar.to(R[(rowbase + r) * a.ordL + ord]) : (V)0;
-            At[r * ms + col] = v;  // At has npad rows; fcol and cflag only n
-            if (col == 0 && r < n) {
-                fcol[r] = v;
-                cflag[r] = v != 0;
-            }
+            At[r * ms + col] = v;  // At has npad rows, rows n .. npad-1 zero
         }
     }
     __syncthreads();
 
-    // Gauss-Jordan, one barrier per column.  While eliminating column c the
-    // threads of column c+1 keep its new values (fcol) and flag the rows that
-    // can be its pivot (nonzero, not a pivot row yet); every wave then finds
-    // the first flagged row with a ballot, so no atomics or extra barrier.
+    // Gauss-Jordan by panels of kPanel columns
     int rank = 0;
-    for (int c = 0; c < m && rank < n; ++c) {
-        const int *cf = cflag + (c & 1) * kMaxDim;
-        int *cfn = cflag + ((c + 1) & 1) * kMaxDim;
-        const V *fc = fcol + (c & 1) * kMaxDim;
-        V *fn = fcol + ((c + 1) & 1) * kMaxDim;
-        const unsigned long long b0 = __ballot(lane < n && cf[lane]);
-        const unsigned long long b1 = __ballot(lane + 64 < n && cf[lane + 64]);
-        const int pr = b0 ? __ffsll(b0) - 1 : (b1 ? 63 + __ffsll(b1) : -1);
-        const bool next = active && col == c + 1;
-        if (pr >= 0) {
-            // identical writes from every lane: no branch on the thread index
-            prow[rank] = pr;
-            pivcol[rank] = c;
-            ispiv[pr] = 1;  // read from the next iteration on (the row is excluded below)
-            colpiv[c] = 1;
-            const V pv = fc[pr];
-            if (active) {
-                const V pcol = At[pr * ms + col];
-                for (int k = 0; k < rpt; ++k) {
-                    const int r = r0 + k * rstep;
-                    const int rr = r < n ? r : n - 1;  // padding rows (zero) read a valid slot
-                    V *x = At + r * ms + col;
-                    const V v = *x;
-                    const V f = r < n ? fc[rr] : (V)0;
-                    const V nv = ar.sub(ar.mul(pv, v), ar.mul(f, pcol));
-                    const V w = r != pr ? nv : v;
-                    *x = w;
-                    if (next && r < n) {
-                        fn[r] = w;
-                        cfn[r] = w != 0 && r != pr && !ispiv[r];
+    const int wave = tid >> 6;
+    for (int c0 = 0; c0 < m && rank < n; c0 += kPanel) {
+        if (wave == 0) {
+            // rows lane and lane + 64 of the panel, their pivot flags and the
+            // coefficients C[.][0..kPanel] of the composition
+            V pn[2][kPanel], cc[2][kPanel + 1];
+            bool isp[2], ok[2];
+            int prs[kPanel] = {0};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int r = lane + 64 * h;
+                ok[h] = r < n;
+                isp[h] = ok[h] && ispiv[r];
+#pragma unroll
+                for (int j = 0; j < kPanel; ++j)
+                    pn[h][j] = ok[h] && c0 + j < m ? At[r * ms + c0 + j] : (V)0;
+                cc[h][0] = ar.one();
+#pragma unroll
+                for (int s2 = 1; s2 <= kPanel; ++s2)
+                    cc[h][s2] = 0;
+            }
+            int nb = 0, mask = 0;
+            // one pivot step on panel column j (a compile-time index: the
+            // convergent ballots and shuffles keep the compiler from unrolling
+            // a loop around them, and the arrays must stay in registers)
+            auto step = [&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                // columns past m, or a full rank already reached, take no pivot
+                const bool open = c0 + j < m && rank + nb < n;
+                const unsigned long long b0 = __ballot(open && ok[0] && !isp[0] && pn[0][j] != 0);
+                const unsigned long long b1 = __ballot(open && ok[1] && !isp[1] && pn[1][j] != 0);
+                if (!b0 && !b1)
+                    return;  // no pivot in this column
+                const int pr = b0 ? __ffsll(b0) - 1 : 63 + __ffsll(b1);
+                const int pl = pr & 63, ph = pr >> 6;
+                // the pivot row's self coefficient moves to slot j + 1 (ph is
+                // uniform: explicit branches keep the arrays in registers)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (lane == pl && ph == h) {
+                        cc[h][j + 1] = cc[h][0];
+                        cc[h][0] = 0;
+                        isp[h] = true;
                     }
                 }
+                // broadcast the pivot row (panel values and coefficients)
+                V qp[kPanel], qc[kPanel + 1];
+#pragma unroll
+                for (int jj = j; jj < kPanel; ++jj)
+                    qp[jj] = __shfl(ph ? pn[1][jj] : pn[0][jj], pl);
+#pragma unroll
+                for (int s2 = 1; s2 <= j + 1; ++s2)
+                    qc[s2] = __shfl(ph ? cc[1][s2] : cc[0][s2], pl);
+                const V pv = qp[j];
+                // Only the nonzero terms are formed: column j of every other
+                // row becomes pv * fr - fr * pv = 0, the pivot row's slot 0 is
+                // zero after the move, and slots past j + 1 are zero in every
+                // row until later steps.
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!ok[h] || lane + 64 * h == pr)
+                        continue;
+                    const V fr = pn[h][j];
+                    pn[h][j] = 0;
+#pragma unroll
+                    for (int jj = j + 1; jj < kPanel; ++jj)
+                        pn[h][jj] = ar.sub(ar.mul(pv, pn[h][jj]), ar.mul(fr, qp[jj]));
+                    cc[h][0] = ar.mul(pv, cc[h][0]);
+#pragma unroll
+                    for (int s2 = 1; s2 <= j + 1; ++s2)
+                        cc[h][s2] = ar.sub(ar.mul(pv, cc[h][s2]), ar.mul(fr, qc[s2]));
+                }
+                prs[j] = pr;
+                mask |= 1 << j;
+                if (lane == 0) {
+                    prow[rank + nb] = pr;
+                    pivcol[rank + nb] = c0 + j;
+                    colpiv[c0 + j] = 1;
+                }
+                ++nb;
+            };
+            static_assert(kPanel == 4, "the panel steps below are written out for 4 columns");
+            step(std::integral_constant<int, 0>{});
+            step(std::integral_constant<int, 1>{});
+            step(std::integral_constant<int, 2>{});
+            step(std::integral_constant<int, 3>{});
+            // publish: coefficients, pivot flags, the pivot rows at the panel's start
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int r = lane + 64 * h;
+                if (ok[h]) {
+#pragma unroll
+                    for (int s2 = 0; s2 <= kPanel; ++s2)
+                        Ck[r * (kPanel + 1) + s2] = cc[h][s2];
+                    ispiv[r] = isp[h] ? 1 : 0;
+                }
             }
-            ++rank;
-        } else if (next) {
+#pragma unroll
+            for (int k = 0; k < kPanel; ++k) {
+                if (mask & (1 << k)) {
+                    const int pk = prs[k];
+                    for (int c2 = lane; c2 < m; c2 += 64)
+                        Rp[k * kMaxDim + c2] = At[pk * ms + c2];
+                }
+            }
+            if (lane == 0) {
+                s_nb = nb;
+                s_mask = mask;
+            }
+        }
+        __syncthreads();
+        const int nb = s_nb, mask = s_mask;
+        if (nb > 0 && active) {
             for (int k = 0; k < rpt; ++k) {
                 const int r = r0 + k * rstep;
                 if (r >= n)
                     break;
-                const V v = At[r * ms + col];
-                fn[r] = v;
-                cfn[r] = v != 0 && !ispiv[r];
+                const V *cr = Ck + r * (kPanel + 1);
+                V *x = At + r * ms + col;
+                V acc = ar.mul(cr[0], *x);
+#pragma unroll
+                for (int q = 0; q < kPanel; ++q)
+                    if (mask & (1 << q))
+                        acc = ar.add(acc, ar.mul(cr[q + 1], Rp[q * kMaxDim + col]));
+                *x = acc;
             }
         }
+        rank += nb;
         __syncthreads();
     }
     // pivots of the reduced echelon form: 1 / At[prow[k]][pivcol[k]]
@@ -565,13 +689,13 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     // column snapshots and inverses, then 9 int arrays (sizes as in the kernel)
     const int rstep = kStepThreads / m;
     const size_t npad = (size_t)((n + rstep - 1) / rstep) * rstep;
-    const size_t lds = (npad * (m | 1) + 4 * kMaxDim) * 8 + 9 * kMaxDim * sizeof(int);
+    const size_t lds = (npad * (m | 1) + (3 + 2 * kPanel) * kMaxDim) * 8 + 7 * kMaxDim * sizeof(int);
     static bool attr = false;
     if (!attr) {
         const size_t npmax = kMaxDim + kStepThreads / kMaxDim - 1;
-        const int maxlds = (int)((npmax * (kMaxDim + 1) + 4 * kMaxDim) * 8 +
-                                 9 * kMaxDim * sizeof(int));
-        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel<PlainArith>,
+        const int maxlds = (int)((npmax * (kMaxDim + 1) + (3 + 2 * kPanel) * kMaxDim) * 8 +
+                                 7 * kMaxDim * sizeof(int));
+        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel<Mont64Arith>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, maxlds),
                 "hipFuncSetAttribute(mbasis_step_kernel)");
         HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel<Mont32Arith>,
@@ -587,7 +711,7 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
                                ord);
             hipLaunchKernelGGL(mbasis_update_kernel<true>, dim3(ugrid), dim3(256), 0, s, a, ord);
         } else {
-            hipLaunchKernelGGL(mbasis_step_kernel<PlainArith>, dim3(1), dim3(kStepThreads), lds, s, a,
+            hipLaunchKernelGGL(mbasis_step_kernel<Mont64Arith>, dim3(1), dim3(kStepThreads), lds, s, a,
                                ord);
             hipLaunchKernelGGL(mbasis_update_kernel<false>, dim3(ugrid), dim3(256), 0, s, a, ord);
         }
