@@ -116,6 +116,31 @@ struct Mont64Arith {
         return u >= p ? u - p : u;
     }
     __device__ V mul(V a, V b) const { return redc(mulhi64(a, b), a * b); }
+    // Sums of up to five products, reduced once: t < 5 p^2 < 2^64 p, so the
+    // REDC quotient is below (5p / 2^64 + 1) p < 2.25 p.
+    struct Acc {
+        uint64_t hi, lo;
+    };
+    __device__ Acc prod(V a, V b) const { return Acc{mulhi64(a, b), a * b}; }
+    __device__ Acc fma(Acc t, V a, V b) const
+    {
+        const uint64_t l = a * b, h = mulhi64(a, b);
+        const uint64_t lo = t.lo + l;
+        return Acc{t.hi + h + (lo < l ? 1 : 0), lo};
+    }
+    __device__ V reduce(Acc t) const
+    {
+        const uint64_t mm = t.lo * pinv;
+        uint64_t u = t.hi + mulhi64(mm, p) + (t.lo != 0 ? 1 : 0);
+        u = u >= p ? u - p : u;
+        return u >= p ? u - p : u;
+    }
+    // pv x - fr q as pv x + fr (p - q) < 2 p^2: one reduction, quotient < 1.5 p
+    __device__ V msub(V pv, V x, V fr, V q) const
+    {
+        const Acc t = fma(prod(pv, x), fr, p - q);
+        return redc(t.hi, t.lo);
+    }
     __device__ V sub(V a, V b) const { return a >= b ? a - b : a + (p - b); }
     __device__ V add(V a, V b) const
     {
@@ -155,6 +180,25 @@ struct Mont32Arith {
         return v < u ? v : u;
     }
     __device__ V mul(V a, V b) const { return redc((uint64_t)a * b); }
+    // Sums of up to five products, reduced once: t < 5 p^2 < 5 2^60, t + mm p
+    // < 9 2^60 < 2^64 and the quotient is below (5p / 2^32 + 1) p < 2.25 p.
+    using Acc = uint64_t;
+    __device__ Acc prod(V a, V b) const { return (uint64_t)a * b; }
+    __device__ Acc fma(Acc t, V a, V b) const { return t + (uint64_t)a * b; }
+    __device__ V reduce(Acc t) const
+    {
+        const uint32_t mm = (uint32_t)t * pinv;
+        uint32_t u = (uint32_t)((t + (uint64_t)mm * p) >> 32);
+        uint32_t v = u - p;
+        u = v < u ? v : u;
+        v = u - p;
+        return v < u ? v : u;
+    }
+    // pv x - fr q as pv x + fr (p - q) < 2 p^2: one reduction, quotient < 2p
+    __device__ V msub(V pv, V x, V fr, V q) const
+    {
+        return redc((uint64_t)pv * x + (uint64_t)fr * (p - q));
+    }
     __device__ V sub(V a, V b) const
     {
         // a - b, plus p when it wraps: min(d, d + p) as unsigned words
@@ -351,11 +395,11 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                     pn[h][j] = 0;
 #pragma unroll
                     for (int jj = j + 1; jj < kPanel; ++jj)
-                        pn[h][jj] = ar.sub(ar.mul(pv, pn[h][jj]), ar.mul(fr, qp[jj]));
+                        pn[h][jj] = ar.msub(pv, pn[h][jj], fr, qp[jj]);
                     cc[h][0] = ar.mul(pv, cc[h][0]);
 #pragma unroll
                     for (int s2 = 1; s2 <= j + 1; ++s2)
-                        cc[h][s2] = ar.sub(ar.mul(pv, cc[h][s2]), ar.mul(fr, qc[s2]));
+                        cc[h][s2] = ar.msub(pv, cc[h][s2], fr, qc[s2]);
                 }
                 prs[j] = pr;
                 mask |= 1 << j;
@@ -404,12 +448,12 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                     break;
                 const V *cr = Ck + r * (kPanel + 1);
                 V *x = At + r * ms + col;
-                V acc = ar.mul(cr[0], *x);
+                typename Arith::Acc acc = ar.prod(cr[0], *x);
 #pragma unroll
                 for (int q = 0; q < kPanel; ++q)
                     if (mask & (1 << q))
-                        acc = ar.add(acc, ar.mul(cr[q + 1], Rp[q * kMaxDim + col]));
-                *x = acc;
+                        acc = ar.fma(acc, cr[q + 1], Rp[q * kMaxDim + col]);
+                *x = ar.reduce(acc);
             }
         }
         rank += nb;
@@ -418,15 +462,26 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // pivots of the reduced echelon form: 1 / At[prow[k]][pivcol[k]]
     for (int k = tid; k < rank; k += kStepThreads)
         inv[k] = ar.inv(At[prow[k] * ms + pivcol[k]]);
-    // permutation list of left_nullspace_compact: pivot columns, then the others
-    for (int i = tid; i < m; i += kStepThreads) {
-        int before = 0;
-        for (int j = 0; j < i; ++j)
-            before += colpiv[j];
-        if (colpiv[i])
-            plist[before] = i;
-        else
-            plist[rank + i - before] = i;
+    // permutation list of left_nullspace_compact: pivot columns, then the
+    // others; the last wave (the inverses run on the first ones) counts the
+    // pivot columns before each column by ballots
+    if (tid >= kStepThreads - 64) {
+        const int l = tid - (kStepThreads - 64);
+        int base = 0;
+#pragma unroll
+        for (int h = 0; h < kMaxDim / 64; ++h) {
+            const int i = l + 64 * h;
+            const bool pc = i < m && colpiv[i];
+            const unsigned long long b = __ballot(pc);
+            const int before = base + __popcll(b & ((1ull << l) - 1));
+            if (i < m) {
+                if (pc)
+                    plist[before] = i;
+                else
+                    plist[rank + i - before] = i;
+            }
+            base += __popcll(b);
+        }
     }
     __syncthreads();
     const int nullity = m - rank;
