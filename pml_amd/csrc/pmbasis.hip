@@ -229,6 +229,16 @@ struct Mont32Arith {
     }
 };
 
+// value of v in lane l (l uniform across the wave)
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ uint64_t lane_value(uint64_t v, int l)
+{
+    return ((uint64_t)lane_value((uint32_t)(v >> 32), l) << 32) | lane_value((uint32_t)v, l);
+}
+
 // ---------------------------------------------------------------------------
 // kernel 1: permutation, left nullspace, step description
 // ---------------------------------------------------------------------------
@@ -244,7 +254,7 @@ struct Mont32Arith {
 // row of panel column j at the panel's start), the same exact residues as the
 // one-column-at-a-time sequence.  All threads then apply that rank-(1 + pivots) combination to the
 // whole matrix: two barriers per panel instead of one per column.
-template <typename Arith>
+template <typename Arith, int H>
 __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, int ord)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -332,13 +342,13 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     const int wave = tid >> 6;
     for (int c0 = 0; c0 < m && rank < n; c0 += kPanel) {
         if (wave == 0) {
-            // rows lane and lane + 64 of the panel, their pivot flags and the
+            // rows lane + 64 h (h < H) of the panel, their pivot flags and the
             // coefficients C[.][0..kPanel] of the composition
-            V pn[2][kPanel], cc[2][kPanel + 1];
-            bool isp[2], ok[2];
+            V pn[H][kPanel], cc[H][kPanel + 1];
+            bool isp[H], ok[H];
             int prs[kPanel] = {0};
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < H; ++h) {
                 const int r = lane + 64 * h;
                 ok[h] = r < n;
                 isp[h] = ok[h] && ispiv[r];
@@ -359,7 +369,8 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 // columns past m, or a full rank already reached, take no pivot
                 const bool open = c0 + j < m && rank + nb < n;
                 const unsigned long long b0 = __ballot(open && ok[0] && !isp[0] && pn[0][j] != 0);
-                const unsigned long long b1 = __ballot(open && ok[1] && !isp[1] && pn[1][j] != 0);
+                const unsigned long long b1 =
+                    H == 2 ? __ballot(open && ok[H - 1] && !isp[H - 1] && pn[H - 1][j] != 0) : 0ull;
                 if (!b0 && !b1)
                     return;  // no pivot in this column
                 const int pr = b0 ? __ffsll(b0) - 1 : 63 + __ffsll(b1);
@@ -367,28 +378,30 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 // the pivot row's self coefficient moves to slot j + 1 (ph is
                 // uniform: explicit branches keep the arrays in registers)
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < H; ++h) {
                     if (lane == pl && ph == h) {
                         cc[h][j + 1] = cc[h][0];
                         cc[h][0] = 0;
                         isp[h] = true;
                     }
                 }
-                // broadcast the pivot row (panel values and coefficients)
+                // broadcast the pivot row (panel values and coefficients): the
+                // pivot lane is uniform, so its registers are read into
+                // scalar registers
                 V qp[kPanel], qc[kPanel + 1];
 #pragma unroll
                 for (int jj = j; jj < kPanel; ++jj)
-                    qp[jj] = __shfl(ph ? pn[1][jj] : pn[0][jj], pl);
+                    qp[jj] = lane_value(H == 2 && ph ? pn[H - 1][jj] : pn[0][jj], pl);
 #pragma unroll
                 for (int s2 = 1; s2 <= j + 1; ++s2)
-                    qc[s2] = __shfl(ph ? cc[1][s2] : cc[0][s2], pl);
+                    qc[s2] = lane_value(H == 2 && ph ? cc[H - 1][s2] : cc[0][s2], pl);
                 const V pv = qp[j];
                 // Only the nonzero terms are formed: column j of every other
                 // row becomes pv * fr - fr * pv = 0, the pivot row's slot 0 is
                 // zero after the move, and slots past j + 1 are zero in every
                 // row until later steps.
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < H; ++h) {
                     if (!ok[h] || lane + 64 * h == pr)
                         continue;
                     const V fr = pn[h][j];
@@ -417,7 +430,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             step(std::integral_constant<int, 3>{});
             // publish: coefficients, pivot flags, the pivot rows at the panel's start
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < H; ++h) {
                 const int r = lane + 64 * h;
                 if (ok[h]) {
 #pragma unroll
@@ -750,24 +763,35 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
         const size_t npmax = kMaxDim + kStepThreads / kMaxDim - 1;
         const int maxlds = (int)((npmax * (kMaxDim + 1) + (3 + 2 * kPanel) * kMaxDim) * 8 +
                                  7 * kMaxDim * sizeof(int));
-        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel<Mont64Arith>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, maxlds),
-                "hipFuncSetAttribute(mbasis_step_kernel)");
-        HIP_TRY(hipFuncSetAttribute((const void *)mbasis_step_kernel<Mont32Arith>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, maxlds),
-                "hipFuncSetAttribute(mbasis_step_kernel)");
+        const void *kernels[] = {(const void *)mbasis_step_kernel<Mont64Arith, 1>,
+                                 (const void *)mbasis_step_kernel<Mont64Arith, 2>,
+                                 (const void *)mbasis_step_kernel<Mont32Arith, 1>,
+                                 (const void *)mbasis_step_kernel<Mont32Arith, 2>};
+        for (const void *k : kernels)
+            HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, maxlds),
+                    "hipFuncSetAttribute(mbasis_step_kernel)");
         attr = true;
     }
     const unsigned ugrid = grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL);
     const bool small = ctx->p < ((uint64_t)1 << 30);
+    // rows of the transposed residual per lane of the composing wave
+    const bool one_row = n <= 64;
     for (int ord = 0; ord < ordL; ++ord) {
         if (small) {
-            hipLaunchKernelGGL(mbasis_step_kernel<Mont32Arith>, dim3(1), dim3(kStepThreads), lds, s, a,
-                               ord);
+            if (one_row)
+                hipLaunchKernelGGL((mbasis_step_kernel<Mont32Arith, 1>), dim3(1), dim3(kStepThreads), lds,
+                                   s, a, ord);
+            else
+                hipLaunchKernelGGL((mbasis_step_kernel<Mont32Arith, 2>), dim3(1), dim3(kStepThreads), lds,
+                                   s, a, ord);
             hipLaunchKernelGGL(mbasis_update_kernel<true>, dim3(ugrid), dim3(256), 0, s, a, ord);
         } else {
-            hipLaunchKernelGGL(mbasis_step_kernel<Mont64Arith>, dim3(1), dim3(kStepThreads), lds, s, a,
-                               ord);
+            if (one_row)
+                hipLaunchKernelGGL((mbasis_step_kernel<Mont64Arith, 1>), dim3(1), dim3(kStepThreads), lds,
+                                   s, a, ord);
+            else
+                hipLaunchKernelGGL((mbasis_step_kernel<Mont64Arith, 2>), dim3(1), dim3(kStepThreads), lds,
+                                   s, a, ord);
             hipLaunchKernelGGL(mbasis_update_kernel<false>, dim3(ugrid), dim3(256), 0, s, a, ord);
         }
     }

@@ -131,3 +131,20 @@ def test_pmbasis_dims_64(pml):
     F = make(rng, "uniform", 64, 32, 200, P30)
     P = check(pml, P30, F, 200, np.zeros(64, dtype=np.int64))
     assert O.is_approximant_basis(P, F, 200, np.zeros(64, dtype=np.int64), P30) == 0
+
+
+@pytest.mark.parametrize("m,n,order,p,kind,shift_name", [
+    (100, 70, 40, P30, "uniform", "increasing"),
+    (128, 100, 36, P60, "uniform", "uniform"),
+    (72, 65, 70, P30, "rkdef", "plateau"),
+    (128, 128, 3, P60, "uniform", "shuffle"),
+    (96, 90, 33, P60, "sparse", "decreasing"),
+])
+def test_pmbasis_wide_residual(pml, m, n, order, p, kind, shift_name):
+    # more than 64 residual columns: the composing wave holds two rows of the
+    # transposed residual per lane
+    rng = np.random.default_rng(m * 1000 + n)
+    F = make(rng, kind, m, n, order, p)
+    shift = SHIFTS[shift_name](m, order).astype(np.int64)
+    P = check(pml, p, F, order, shift)
+    assert O.is_approximant_basis(P, F, order, shift, p) == 0
