@@ -94,6 +94,43 @@ __device__ uint64_t invmod_g(uint64_t a, const ModConst &mc)
     return r;
 }
 
+// Inverse of a in [1, p) modulo a prime p < 2^62 by the extended Euclidean
+// algorithm, r_i = s_i a (mod p), with the quotients estimated in double
+// precision.  f = fl(r0) / fl(r1) has relative error below 2^-51, so for
+// f < 2^49 trunc(f) is within one of the quotient and one correction step
+// fixes it; larger quotients (r1 tiny) are refined once on the remainder,
+// which is then below (2^11 + 1) r1 in magnitude.  The cofactors stay below p
+// in magnitude.  About 0.84 ln p division steps on average against ~log2 p
+// dependent squarings for Fermat's a^(p-2).
+__device__ uint64_t inv_euclid(uint64_t a, uint64_t p)
+{
+    uint64_t r0 = p, r1 = a;
+    int64_t s0 = 0, s1 = 1;
+    while (r1 != 0) {
+        const double f = (double)r0 / (double)r1;
+        uint64_t q = (uint64_t)f;
+        int64_t rem = (int64_t)(r0 - q * r1);
+        if (f >= 0x1p49) {
+            const int64_t q2 = (int64_t)((double)rem / (double)r1);
+            q += (uint64_t)q2;
+            rem -= q2 * (int64_t)r1;
+        }
+        if (rem < 0) {
+            rem += (int64_t)r1;
+            --q;
+        } else if ((uint64_t)rem >= r1) {
+            rem -= (int64_t)r1;
+            ++q;
+        }
+        const int64_t s2 = s0 - (int64_t)q * s1;
+        r0 = r1;
+        r1 = (uint64_t)rem;
+        s0 = s1;
+        s1 = s2;
+    }
+    return s0 < 0 ? (uint64_t)(s0 + (int64_t)p) : (uint64_t)s0;
+}
+
 // Arithmetic of the step kernel (p an odd prime below 2^62).  Values in LDS
 // are Montgomery residues: x 2^32 mod p for p < 2^30 (Mont32: one 32x32 -> 64
 // product, one low product and one multiply-add per reduction), x 2^64 mod p
@@ -150,19 +187,8 @@ struct Mont64Arith {
     __device__ V one() const { return r1; }
     __device__ V to(uint64_t x) const { return mul(x, r2); }
     __device__ uint64_t from(V x) const { return redc(0, x); }
-    __device__ V inv(V a) const
-    {
-        // Fermat a^(p-2), in the Montgomery domain (r1 is 1 there)
-        uint64_t e = p - 2;
-        V r = r1, b = a;
-        while (e) {
-            if (e & 1)
-                r = mul(r, b);
-            b = mul(b, b);
-            e >>= 1;
-        }
-        return r;
-    }
+    // a = x R: the plain inverse of a is x^-1 R^-1, and x^-1 R = mul(mul(a^-1, R^2), R^2)
+    __device__ V inv(V a) const { return mul(mul(inv_euclid(a, p), r2), r2); }
 };
 
 struct Mont32Arith {
@@ -229,6 +255,14 @@ struct Mont32Arith {
     }
 };
 
+// workgroup barrier ordering LDS only: global loads in flight stay in flight
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // value of v in lane l (l uniform across the wave)
 __device__ __forceinline__ uint32_t lane_value(uint32_t v, int l)
 {
@@ -269,35 +303,39 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     int64_t *sh = reinterpret_cast<int64_t *>(smem);             // kMaxDim
     V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // npad x ms
     V *inv = At + npad * ms;                                     // kMaxDim
-    V *Ck = inv + kMaxDim;                                       // kMaxDim x (kPanel + 1)
-    V *Rp = Ck + kMaxDim * (kPanel + 1);                         // kPanel x kMaxDim
-    int *perm = reinterpret_cast<int *>(Rp + kPanel * kMaxDim);  // kMaxDim
+    V *Ck = inv + kMaxDim;                                            // 2 x kMaxDim x (kPanel + 1)
+    int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxDim * (kPanel + 1));  // kMaxDim
     int *pivcol = perm + kMaxDim;                                // kMaxDim
     int *plist = pivcol + kMaxDim;                               // kMaxDim
     int *prow = plist + kMaxDim;                                 // kMaxDim
     int *ispiv = prow + kMaxDim;                                 // kMaxDim (rows of At)
     int *colpiv = ispiv + kMaxDim;                               // kMaxDim (columns of At)
     int *pos = colpiv + kMaxDim;                                 // kMaxDim
-    __shared__ int s_state[2];
-    __shared__ int s_nb, s_mask;
+    __shared__ int s_nb[2], s_mask[2], s_prs[2][kPanel];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint64_t p = a.mc.p;
 
-    if (tid == 0) {
-        // control words: independent loads first, then the updates
-        int *c = a.ctl;
-        const int flip = c[C_FLIP], cur = c[C_CUR], len = c[C_LEN], grow = c[C_GROW],
-                  done = c[C_DONE];
-        const int ncur = flip ? cur ^ 1 : cur;
-        c[C_CUR] = ncur;
-        c[C_FLIP] = 0;
-        c[C_LEN] = len + grow;
-        c[C_GROW] = 0;
-        if (done)
-            c[C_MODE] = 0;
-        s_state[0] = done;
-        s_state[1] = ncur;
+    // Control words: every thread reads them, so the residual loads start at
+    // once; thread 0 writes them back only after full barriers.
+    int *c = a.ctl;
+    const int flip = c[C_FLIP], cur0 = c[C_CUR], done = c[C_DONE];
+    const int cur = flip ? cur0 ^ 1 : cur0;
+    const uint64_t *R = a.R[cur];
+    // this thread's elements: row col of the (unpermuted) residual, columns
+    // r0 + k rstep (k < rpt), stored to column pos[col] of At once the
+    // permutation is known; the first kPre are loaded before the sort
+    const int col = tid % m;
+    const int r0 = tid / m;
+    const bool active = r0 < rstep;
+    constexpr int kPre = 4;
+    uint64_t pre[kPre];
+    if (!done && active) {
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+            const int r = r0 + k * rstep;
+            pre[k] = k < rpt && r < n ? R[((int64_t)col * n + r) * a.ordL + ord] : 0;
+        }
     }
     for (int i = tid; i < m; i += kStepThreads) {
         sh[i] = a.shift[i];
@@ -306,11 +344,19 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     }
     for (int r = tid; r < kMaxDim; r += kStepThreads)
         ispiv[r] = 0;
-    __syncthreads();
-    if (s_state[0])
+    if (done) {
+        __syncthreads();
+        if (tid == 0) {
+            const int len = c[C_LEN] + c[C_GROW];
+            c[C_CUR] = cur;
+            c[C_FLIP] = 0;
+            c[C_LEN] = len;
+            c[C_GROW] = 0;
+            c[C_MODE] = 0;
+        }
         return;
-    const int cur = s_state[1];
-    const uint64_t *R = a.R[cur];
+    }
+    lds_barrier();
 
     // stable permutation sorting (shift, index): pos[i] = #{j : (sh_j, j) < (sh_i, i)}
     for (int idx = tid; idx < m * m; idx += kStepThreads) {
@@ -319,29 +365,36 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         if ((sj < si) || (sj == si && j < i))
             atomicAdd(&pos[i], 1);
     }
-    __syncthreads();
+    lds_barrier();
     for (int i = tid; i < m; i += kStepThreads)
         perm[pos[i]] = i;
-    __syncthreads();
-    // this thread's elements of At: column col, rows r0 + k rstep, k < rpt
-    const int col = tid % m;
-    const int r0 = tid / m;
-    const bool active = r0 < rstep;
     if (active) {
-        const int64_t rowbase = (int64_t)perm[col] * n;
-        for (int k = 0; k < rpt; ++k) {
+        // At has npad rows, rows n .. npad-1 zero
+        const int pc = pos[col];
+#pragma unroll
+        for (int k = 0; k < kPre; ++k)
+            if (k < rpt)
+                At[(r0 + k * rstep) * ms + pc] = ar.to(pre[k]);
+        for (int k = kPre; k < rpt; ++k) {
             const int r = r0 + k * rstep;
-            const V v = r < n ? ar.to(R[(rowbase + r) * a.ordL + ord]) : (V)0;
-            At[r * ms + col] = v;  // At has npad rows, rows n .. npad-1 zero
+            At[r * ms + pc] = r < n ? ar.to(R[((int64_t)col * n + r) * a.ordL + ord]) : (V)0;
         }
     }
     __syncthreads();
 
-    // Gauss-Jordan by panels of kPanel columns
+    // Gauss-Jordan by panels of kPanel columns.  Wave w owns the columns
+    // [w cw, (w + 1) cw) of At and applies every panel's combination to them;
+    // the owner of the next panel then composes it from its own updated
+    // columns (the LDS operations of one wave take effect in program order)
+    // while the other waves are still applying: one barrier per panel, with
+    // the coefficients double-buffered.
     int rank = 0;
     const int wave = tid >> 6;
-    for (int c0 = 0; c0 < m && rank < n; c0 += kPanel) {
-        if (wave == 0) {
+    const int cw = m <= 16 * kPanel ? kPanel : 2 * kPanel;  // 16 waves cover kMaxDim columns
+    // compose the panel of columns c0 .. c0 + kPanel - 1 into buffer buf
+    auto compose = [&](const int c0, const int buf) {
+        __builtin_amdgcn_s_setprio(2);
+        {
             // rows lane + 64 h (h < H) of the panel, their pivot flags and the
             // coefficients C[.][0..kPanel] of the composition
             V pn[H][kPanel], cc[H][kPanel + 1];
@@ -435,43 +488,61 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 if (ok[h]) {
 #pragma unroll
                     for (int s2 = 0; s2 <= kPanel; ++s2)
-                        Ck[r * (kPanel + 1) + s2] = cc[h][s2];
+                        Ck[(buf * kMaxDim + r) * (kPanel + 1) + s2] = cc[h][s2];
                     ispiv[r] = isp[h] ? 1 : 0;
                 }
             }
-#pragma unroll
-            for (int k = 0; k < kPanel; ++k) {
-                if (mask & (1 << k)) {
-                    const int pk = prs[k];
-                    for (int c2 = lane; c2 < m; c2 += 64)
-                        Rp[k * kMaxDim + c2] = At[pk * ms + c2];
-                }
-            }
             if (lane == 0) {
-                s_nb = nb;
-                s_mask = mask;
-            }
-        }
-        __syncthreads();
-        const int nb = s_nb, mask = s_mask;
-        if (nb > 0 && active) {
-            for (int k = 0; k < rpt; ++k) {
-                const int r = r0 + k * rstep;
-                if (r >= n)
-                    break;
-                const V *cr = Ck + r * (kPanel + 1);
-                V *x = At + r * ms + col;
-                typename Arith::Acc acc = ar.prod(cr[0], *x);
+                s_nb[buf] = nb;
+                s_mask[buf] = mask;
 #pragma unroll
-                for (int q = 0; q < kPanel; ++q)
-                    if (mask & (1 << q))
-                        acc = ar.fma(acc, cr[q + 1], Rp[q * kMaxDim + col]);
-                *x = ar.reduce(acc);
+                for (int k = 0; k < kPanel; ++k)
+                    s_prs[buf][k] = prs[k];
             }
         }
+        __builtin_amdgcn_s_setprio(0);
+    };
+    // apply the combination in buffer buf to this wave's columns: the pivot
+    // rows' values are read before any row is written
+    auto apply = [&](const int buf, const int mask) {
+        const int G = 64 / cw;
+        const int cl = lane % cw, g = lane / cw;
+        const int cg = wave * cw + cl;
+        if (cg >= m)
+            return;
+        V pvr[kPanel];
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q)
+            pvr[q] = mask & (1 << q) ? At[s_prs[buf][q] * ms + cg] : (V)0;
+        const V *C = Ck + buf * kMaxDim * (kPanel + 1);
+        for (int r = g; r < n; r += G) {
+            const V *cr = C + r * (kPanel + 1);
+            V *x = At + r * ms + cg;
+            typename Arith::Acc acc = ar.prod(cr[0], *x);
+#pragma unroll
+            for (int q = 0; q < kPanel; ++q)
+                if (mask & (1 << q))
+                    acc = ar.fma(acc, cr[q + 1], pvr[q]);
+            *x = ar.reduce(acc);
+        }
+    };
+    if (wave == 0)
+        compose(0, 0);
+    __syncthreads();
+    for (int k = 0;; ++k) {
+        const int buf = k & 1;
+        const int nb = s_nb[buf], mask = s_mask[buf];
+        if (nb > 0)
+            apply(buf, mask);
         rank += nb;
+        const int c1 = (k + 1) * kPanel;
+        if (c1 >= m || rank >= n)
+            break;
+        if (wave == c1 / cw)
+            compose(c1, buf ^ 1);
         __syncthreads();
     }
+    __syncthreads();
     // pivots of the reduced echelon form: 1 / At[prow[k]][pivcol[k]]
     for (int k = tid; k < rank; k += kStepThreads)
         inv[k] = ar.inv(At[prow[k] * ms + pivcol[k]]);
@@ -517,12 +588,13 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             a.shift[i] = sh[i] + (a.ordL - ord);
     }
     if (tid == 0) {
-        int *c = a.ctl;
-        const int len = c[C_LEN];
+        const int len = c[C_LEN] + c[C_GROW];
+        c[C_CUR] = cur;
+        c[C_GROW] = 0;
+        c[C_LEN] = len;
         c[C_MODE] = mode;
         c[C_RANK] = rank;
-        if (mode != 0)
-            c[C_FLIP] = 1;
+        c[C_FLIP] = mode != 0 ? 1 : 0;
         if (mode == 2) {
             c[C_SHIFT] = a.ordL - ord;
             c[C_LOLD] = len;
@@ -754,14 +826,14 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
                        dim3(256), 0, s, a, F, strideF, lt);
     HIP_TRY(hipGetLastError(), "leaf_init_kernel");
     // LDS of the step kernel: shifts, At padded to a multiple of the row step,
-    // column snapshots and inverses, then 9 int arrays (sizes as in the kernel)
+    // inverses, two coefficient buffers, then 7 int arrays (sizes as in the kernel)
     const int rstep = kStepThreads / m;
     const size_t npad = (size_t)((n + rstep - 1) / rstep) * rstep;
-    const size_t lds = (npad * (m | 1) + (3 + 2 * kPanel) * kMaxDim) * 8 + 7 * kMaxDim * sizeof(int);
+    const size_t lds = (npad * (m | 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 + 7 * kMaxDim * sizeof(int);
     static bool attr = false;
     if (!attr) {
         const size_t npmax = kMaxDim + kStepThreads / kMaxDim - 1;
-        const int maxlds = (int)((npmax * (kMaxDim + 1) + (3 + 2 * kPanel) * kMaxDim) * 8 +
+        const int maxlds = (int)((npmax * (kMaxDim + 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 +
                                  7 * kMaxDim * sizeof(int));
         const void *kernels[] = {(const void *)mbasis_step_kernel<Mont64Arith, 1>,
                                  (const void *)mbasis_step_kernel<Mont64Arith, 2>,
