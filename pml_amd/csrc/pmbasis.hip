@@ -316,10 +316,18 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     const int lane = tid & 63;
     const uint64_t p = a.mc.p;
 
-    // Control words: every thread reads them, so the residual loads start at
-    // once; thread 0 writes them back only after full barriers.
+    // Shifts and control words first, the shifts stored before the residual
+    // loads are issued (a wait on a vector load also waits on those issued
+    // before it); every thread reads the control words, so the residual loads
+    // start at once, and thread 0 writes them back only after full barriers.
+    const int64_t shv = tid < m ? a.shift[tid] : 0;  // m <= kMaxDim <= kStepThreads
     int *c = a.ctl;
     const int flip = c[C_FLIP], cur0 = c[C_CUR], done = c[C_DONE];
+    if (tid < m) {
+        sh[tid] = shv;
+        pos[tid] = 0;
+        colpiv[tid] = 0;
+    }
     const int cur = flip ? cur0 ^ 1 : cur0;
     const uint64_t *R = a.R[cur];
     // this thread's elements: row col of the (unpermuted) residual, columns
@@ -336,11 +344,6 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             const int r = r0 + k * rstep;
             pre[k] = k < rpt && r < n ? R[((int64_t)col * n + r) * a.ordL + ord] : 0;
         }
-    }
-    for (int i = tid; i < m; i += kStepThreads) {
-        sh[i] = a.shift[i];
-        pos[i] = 0;
-        colpiv[i] = 0;
     }
     for (int r = tid; r < kMaxDim; r += kStepThreads)
         ispiv[r] = 0;
@@ -608,60 +611,61 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
 // kernel 2: apply the step to appbas (and to the residual series)
 // ---------------------------------------------------------------------------
 // base + sum_r X[t][r] * src[row piv[r], column j, coefficient d].  The
-// loads of a group are issued before its products so that they overlap.
+// loads of a group (one L2 round trip) are issued before its products; the
+// row list piv sits in LDS.
 template <bool SMALL>
-__device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArgs &a, int t, int rank,
-                                              int stride, int cols, int j, int d, uint64_t base)
+__device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArgs &a, const int *piv, int t,
+                                              int rank, int stride, int cols, int j, int d, uint64_t base)
 {
     const uint64_t *xr = a.X + (size_t)t * a.m;
     const uint64_t *sj = src + (size_t)j * stride + d;
     const size_t rstride = (size_t)cols * stride;
     if constexpr (SMALL) {
-        // p < 2^30: a product is below 2^60, 8 of them plus a residue below 2^64
-        constexpr int G = 8;
+        // p < 2^30: 16 products stay below 16 (p - 1)^2 < 2^64 - 2^35, so a
+        // residue can be added before the reduction; words below 2^30 are
+        // read through their low halves
+        constexpr int G = 16;
         uint64_t acc = base;
         int r = 0;
-        for (; r + G <= rank; r += G) {
-            uint64_t sv[G], xv[G];
+        for (; r + G <= rank; r += G) {  // full groups: unguarded loads
+            uint32_t sv[G], xv[G];
 #pragma unroll
             for (int u = 0; u < G; ++u) {
-                sv[u] = sj[(size_t)a.piv[r + u] * rstride];
-                xv[u] = xr[r + u];
+                sv[u] = (uint32_t)sj[(size_t)piv[r + u] * rstride];
+                xv[u] = (uint32_t)xr[r + u];
             }
             uint64_t part = 0;
 #pragma unroll
             for (int u = 0; u < G; ++u)
-                part += xv[u] * sv[u];
+                part += (uint64_t)xv[u] * sv[u];
             acc = barrett(acc + part, a.mc.bm, a.mc.p);
         }
         uint64_t part = 0;
         for (; r < rank; ++r)
-            part += xr[r] * sj[(size_t)a.piv[r] * rstride];
+            part += (uint64_t)(uint32_t)xr[r] * (uint32_t)sj[(size_t)piv[r] * rstride];
         return barrett(acc + part, a.mc.bm, a.mc.p);
     } else {
-        // 64-bit words: exact 128-bit sums, folded every 15 products (each < 2^124)
-        constexpr int G = 5;
-        uint64_t hi = 0, lo = base;
-        int r = 0, cnt = 0;
-        for (; r + G <= rank; r += G) {
+        // 64-bit words: exact 128-bit sums of 15 products (each < 2^124) and
+        // a residue, folded once per group
+        constexpr int G = 15;
+        uint64_t acc = base;
+        int r = 0;
+        for (; r + G <= rank; r += G) {  // full groups: unguarded loads
             uint64_t sv[G], xv[G];
 #pragma unroll
             for (int u = 0; u < G; ++u) {
-                sv[u] = sj[(size_t)a.piv[r + u] * rstride];
+                sv[u] = sj[(size_t)piv[r + u] * rstride];
                 xv[u] = xr[r + u];
             }
+            uint64_t hi = 0, lo = acc;
 #pragma unroll
             for (int u = 0; u < G; ++u)
                 mac128(hi, lo, xv[u], sv[u]);
-            cnt += G;
-            if (cnt >= 15) {
-                lo = reduce128(hi, lo, a.mc);
-                hi = 0;
-                cnt = 0;
-            }
+            acc = reduce128(hi, lo, a.mc);
         }
+        uint64_t hi = 0, lo = acc;  // fewer than 15 products left
         for (; r < rank; ++r)
-            mac128(hi, lo, xr[r], sj[(size_t)a.piv[r] * rstride]);
+            mac128(hi, lo, xr[r], sj[(size_t)piv[r] * rstride]);
         return reduce128(hi, lo, a.mc);
     }
 }
@@ -689,6 +693,10 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
         }
         return;
     }
+    __shared__ int piv[kMaxDim];
+    for (int i = threadIdx.x; i < m; i += blockDim.x)
+        piv[i] = a.piv[i];
+    __syncthreads();
     const int rank = a.ctl[C_RANK];
     const int L = a.ctl[C_LEN];
     const unsigned L1 = (unsigned)(L + 1 <= cap ? L + 1 : cap);
@@ -700,7 +708,7 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
         const int ri = a.rowinfo[i];
         uint64_t v;
         if (ri >= 0) {
-            v = d < L ? row_combo<SMALL>(src, a, ri, rank, cap, m, j, d, src[(size_t)e * cap + d]) : 0;
+            v = d < L ? row_combo<SMALL>(src, a, piv, ri, rank, cap, m, j, d, src[(size_t)e * cap + d]) : 0;
         } else {
             v = d >= 1 ? src[(size_t)e * cap + d - 1] : 0;
             if (d == L && v != 0)
@@ -721,7 +729,7 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
         const int i = (int)(e / (unsigned)n), j = (int)(e - (unsigned)i * n);
         const int ri = a.rowinfo[i];
         rd[(size_t)e * a.ordL + d] =
-            ri >= 0 ? row_combo<SMALL>(rs, a, ri, rank, a.ordL, n, j, d, rs[(size_t)e * a.ordL + d])
+            ri >= 0 ? row_combo<SMALL>(rs, a, piv, ri, rank, a.ordL, n, j, d, rs[(size_t)e * a.ordL + d])
                     : rs[(size_t)e * a.ordL + d - 1];
     }
 }
