@@ -145,6 +145,8 @@ def lib():
             L.pmlg_polmat_mulmid_entries.restype = st
             L.pmlg_polmat_mulmid_entries.argtypes = [vp, pp, _i64p, pp, _i64p, pp, _i64p, i64,
                                                      i64, i64, i64, i64, c_int]
+        L.pmlg_pmbasis_dev.restype = st
+        L.pmlg_pmbasis_dev.argtypes = [vp, vp, i64, _i64p, _i64p, vp, i64, i64, i64, i64, c_int, vp]
         if hasattr(L, "pmlg_kernel_basis"):
             L.pmlg_kernel_basis.restype = st
             L.pmlg_kernel_basis.argtypes = [vp, _u64p, i64, _i64p, _i64p, _i64p, _i64p, _u64p, i64,
@@ -295,6 +297,19 @@ class Context:
         _check(fn(self.handle, _p(P), ctypes.byref(L), sh.ctypes.data_as(_i64p), _p(F), m, n,
                   lf, order, cap, *extra))
         return P[:, :, : L.value].copy(), sh
+
+    def pmbasis_dev(self, dP, capP, shift, dF, m, n, lenF, order, method="auto", stream=None):
+        """PM-Basis on device buffers (pmlg_pmbasis_dev): F is m x n x lenF and P
+        m x m x capP (capP >= order + 1) in device memory, entry-major; `shift`
+        (int64 numpy array) is updated in place.  Returns the basis length."""
+        sh = np.ascontiguousarray(shift, dtype=np.int64)
+        if sh is not shift:
+            raise ValueError("shift must be a contiguous int64 array (updated in place)")
+        L = ctypes.c_int64(0)
+        _check(lib().pmlg_pmbasis_dev(self.handle, int(dP), int(capP), ctypes.byref(L),
+                                      sh.ctypes.data_as(_i64p), int(dF), m, n, lenF, order,
+                                      _method(method), stream))
+        return L.value
 
     def pmbasis(self, F, order, shift, method="auto"):
         """Shift-ordered weak Popov approximant basis (nmod_poly_mat_pmbasis)."""

@@ -985,19 +985,31 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
     HIP_TRY(hipMemcpyAsync(shift, dShift, (size_t)m * 8, hipMemcpyDeviceToHost, s), "D2H(shift)");
     const int64_t L = std::min<int64_t>(P.len, capP);
     if (hP) {
-        std::vector<uint64_t> tmp((size_t)(m * m * P.cap));
-        HIP_TRY(hipMemcpyAsync(tmp.data(), P.d, tmp.size() * 8, hipMemcpyDeviceToHost, s), "D2H(P)");
+        // entries packed to the computed length on the device, then read back
+        const uint64_t *src = P.d;
+        if (P.cap != L && L > 0) {
+            uint64_t *packed = ar.alloc((size_t)(m * m * L));
+            if (packed) {
+                hipLaunchKernelGGL(copy_poly_kernel, dim3(grid_for(m * m * L)), dim3(256), 0, s, packed,
+                                   (int64_t)L, P.d, P.cap, (int64_t)(m * m), L);
+                HIP_TRY(hipGetLastError(), "copy_poly_kernel");
+                src = packed;
+            }
+        }
+        const int64_t scap = src == P.d ? P.cap : L;
+        std::vector<uint64_t> tmp((size_t)(m * m * scap));
+        HIP_TRY(hipMemcpyAsync(tmp.data(), src, tmp.size() * 8, hipMemcpyDeviceToHost, s), "D2H(P)");
         HIP_TRY(hipStreamSynchronize(s), "sync");
         // normalised length (the basis has degree at most order)
         int64_t Ln = 0;
         for (int64_t e = 0; e < m * m; ++e)
             for (int64_t d = L - 1; d >= Ln; --d)
-                if (tmp[(size_t)(e * P.cap + d)]) {
+                if (tmp[(size_t)(e * scap + d)]) {
                     Ln = d + 1;
                     break;
                 }
         for (int64_t e = 0; e < m * m; ++e) {
-            std::memcpy(hP + e * capP, tmp.data() + e * P.cap, (size_t)Ln * 8);
+            std::memcpy(hP + e * capP, tmp.data() + e * scap, (size_t)Ln * 8);
             std::memset(hP + e * capP + Ln, 0, (size_t)(capP - Ln) * 8);
         }
         *lenP = Ln;

@@ -148,3 +148,33 @@ def test_pmbasis_wide_residual(pml, m, n, order, p, kind, shift_name):
     shift = SHIFTS[shift_name](m, order).astype(np.int64)
     P = check(pml, p, F, order, shift)
     assert O.is_approximant_basis(P, F, order, shift, p) == 0
+
+
+@pytest.mark.parametrize("m,n,lenF,order,p", [
+    (6, 3, 40, 40, P30),      # one leaf level below the root
+    (9, 4, 150, 100, P60),    # F longer than the order
+    (5, 5, 20, 70, P30),      # F shorter than the order
+    (64, 32, 200, 200, P60),
+])
+def test_pmbasis_device_resident(pml, m, n, lenF, order, p):
+    # pmlg_pmbasis_dev on device buffers gives the host entry point's basis
+    rng = np.random.default_rng(m * 7 + order)
+    F = rng.integers(0, p, (m, n, lenF), dtype=np.uint64)
+    shift = rng.integers(-5, 20, m).astype(np.int64)
+    ctx = ctx_for(pml, p)
+    P, sh = ctx.pmbasis(F, order, shift)
+    cap = order + 1
+    dF = pml.DeviceArray(ctx, F.nbytes)
+    dF.upload(F)
+    dP = pml.DeviceArray(ctx, m * m * cap * 8)
+    dP.upload(np.full((m, m, cap), np.uint64(2**64 - 1), dtype=np.uint64))
+    sh_d = shift.copy()
+    L = ctx.pmbasis_dev(dP.ptr, cap, sh_d, dF.ptr, m, n, lenF, order)
+    # the device length is the computed one: the host entry point's basis
+    # followed by zero coefficients
+    Lh = P.shape[2]
+    assert Lh <= L <= cap
+    D = dP.download(np.empty((m, m, cap), dtype=np.uint64))[:, :, :L]
+    assert np.array_equal(D[:, :, :Lh], P)
+    assert not D[:, :, Lh:].any()
+    assert np.array_equal(sh_d, sh)
