@@ -40,6 +40,7 @@ namespace {
 constexpr int kStepThreads = 1024;
 constexpr int kMaxDim = 128;  // m, n limits of the device M-Basis step
 constexpr int kPanel = 4;     // Gauss-Jordan columns composed per pair of barriers
+constexpr int kXPad = 32;     // words after X read (and discarded) by the last update group
 
 // control words of one leaf
 enum Ctl {
@@ -610,9 +611,12 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
 // ---------------------------------------------------------------------------
 // kernel 2: apply the step to appbas (and to the residual series)
 // ---------------------------------------------------------------------------
-// base + sum_r X[t][r] * src[row piv[r], column j, coefficient d].  The
-// loads of a group (one L2 round trip) are issued before its products; the
-// row list piv sits in LDS.
+// base + sum_r X[t][r] * src[row piv[r], column j, coefficient d].  All
+// loads of a group of terms (32 words below 2^30, or 16 64-bit words: one L2
+// round trip) are issued before its products, with no branch between them:
+// terms past the rank read row piv[r] of the LDS row list (padded with row 0
+// up to kMaxDim) and X words past the row (X carries kXPad words of padding),
+// and their X coefficient is zeroed by a select after the load.
 template <bool SMALL>
 __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArgs &a, const int *piv, int t,
                                               int rank, int stride, int cols, int j, int d, uint64_t base)
@@ -620,42 +624,38 @@ __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArg
     const uint64_t *xr = a.X + (size_t)t * a.m;
     const uint64_t *sj = src + (size_t)j * stride + d;
     const size_t rstride = (size_t)cols * stride;
-    if constexpr (SMALL) {
-        // p < 2^30: 16 products stay below 16 (p - 1)^2 < 2^64 - 2^35, so a
-        // residue can be added before the reduction; words below 2^30 are
-        // read through their low halves
-        constexpr int G = 16;
-        uint64_t acc = base;
-        int r = 0;
-        for (; r + G <= rank; r += G) {  // full groups: unguarded loads
+    constexpr int G = SMALL ? 32 : 16;
+    uint64_t acc = base;
+    for (int r = 0; r < rank; r += G) {
+        if constexpr (SMALL) {
+            // p < 2^30: words are read through their low halves; 16 products
+            // stay below 16 (p - 1)^2 < 2^64 - 2^35, so each half-group is
+            // summed with a residue before one Barrett reduction
             uint32_t sv[G], xv[G];
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 sv[u] = (uint32_t)sj[(size_t)piv[r + u] * rstride];
-                xv[u] = (uint32_t)xr[r + u];
+                const uint32_t x = (uint32_t)xr[r + u];
+                xv[u] = r + u < rank ? x : 0u;
             }
-            uint64_t part = 0;
 #pragma unroll
-            for (int u = 0; u < G; ++u)
-                part += (uint64_t)xv[u] * sv[u];
-            acc = barrett(acc + part, a.mc.bm, a.mc.p);
-        }
-        uint64_t part = 0;
-        for (; r < rank; ++r)
-            part += (uint64_t)(uint32_t)xr[r] * (uint32_t)sj[(size_t)piv[r] * rstride];
-        return barrett(acc + part, a.mc.bm, a.mc.p);
-    } else {
-        // 64-bit words: exact 128-bit sums of 15 products (each < 2^124) and
-        // a residue, folded once per group
-        constexpr int G = 15;
-        uint64_t acc = base;
-        int r = 0;
-        for (; r + G <= rank; r += G) {  // full groups: unguarded loads
+            for (int h = 0; h < G; h += 16) {
+                uint64_t part = 0;
+#pragma unroll
+                for (int u = h; u < h + 16; ++u)
+                    part += (uint64_t)xv[u] * sv[u];
+                acc = barrett(acc + part, a.mc.bm, a.mc.p);
+            }
+        } else {
+            // 64-bit words (p < 2^62): an exact 128-bit sum of 16 products
+            // and a residue, 16 (p - 1)^2 + p < 2^128 - 2^67, folded once (the
+            // 128-bit products are the bound here, not the loads)
             uint64_t sv[G], xv[G];
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 sv[u] = sj[(size_t)piv[r + u] * rstride];
-                xv[u] = xr[r + u];
+                const uint64_t x = xr[r + u];
+                xv[u] = r + u < rank ? x : 0;
             }
             uint64_t hi = 0, lo = acc;
 #pragma unroll
@@ -663,11 +663,8 @@ __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArg
                 mac128(hi, lo, xv[u], sv[u]);
             acc = reduce128(hi, lo, a.mc);
         }
-        uint64_t hi = 0, lo = acc;  // fewer than 15 products left
-        for (; r < rank; ++r)
-            mac128(hi, lo, xr[r], sj[(size_t)piv[r] * rstride]);
-        return reduce128(hi, lo, a.mc);
     }
+    return acc;
 }
 
 template <bool SMALL>
@@ -693,44 +690,50 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
         }
         return;
     }
-    __shared__ int piv[kMaxDim];
-    for (int i = threadIdx.x; i < m; i += blockDim.x)
-        piv[i] = a.piv[i];
+    __shared__ int piv[kMaxDim], rinfo[kMaxDim];
+    for (int i = threadIdx.x; i < kMaxDim; i += blockDim.x) {
+        piv[i] = i < m ? a.piv[i] : 0;  // groups read up to kMaxDim entries
+        rinfo[i] = i < m ? a.rowinfo[i] : 0;
+    }
     __syncthreads();
     const int rank = a.ctl[C_RANK];
     const int L = a.ctl[C_LEN];
     const unsigned L1 = (unsigned)(L + 1 <= cap ? L + 1 : cap);
     const unsigned items = (unsigned)(m * m) * L1;
-    for (unsigned idx = t0; idx < items; idx += tstride) {
-        const unsigned e = idx / L1;
-        const int d = (int)(idx - e * L1);
-        const int i = (int)(e / (unsigned)m), j = (int)(e - (unsigned)i * m);
-        const int ri = a.rowinfo[i];
-        uint64_t v;
-        if (ri >= 0) {
-            v = d < L ? row_combo<SMALL>(src, a, piv, ri, rank, cap, m, j, d, src[(size_t)e * cap + d]) : 0;
-        } else {
-            v = d >= 1 ? src[(size_t)e * cap + d - 1] : 0;
-            if (d == L && v != 0)
-                atomicOr(&a.ctl[C_GROW], 1);
-        }
-        dst[(size_t)e * cap + d] = v;
-    }
     // residual series: coefficients ord+1 .. ordL-1
     const uint64_t *rs = a.R[cur];
     uint64_t *rd = a.R[cur ^ 1];
-    const unsigned span = (unsigned)(a.ordL - ord - 1);
-    if ((int)span <= 0)
-        return;
+    const int spn = a.ordL - ord - 1;
+    const unsigned span = spn > 0 ? (unsigned)spn : 0u;
     const unsigned ritems = (unsigned)(m * n) * span;
-    for (unsigned idx = t0; idx < ritems; idx += tstride) {
-        const unsigned e = idx / span;
-        const int d = ord + 1 + (int)(idx - e * span);
-        const int i = (int)(e / (unsigned)n), j = (int)(e - (unsigned)i * n);
-        const int ri = a.rowinfo[i];
-        rd[(size_t)e * a.ordL + d] =
-            ri >= 0 ? row_combo<SMALL>(rs, a, piv, ri, rank, a.ordL, n, j, d, rs[(size_t)e * a.ordL + d])
-                    : rs[(size_t)e * a.ordL + d - 1];
+    // one index space, basis items then residual items: with the grid sized
+    // for both, a thread has one item
+    for (unsigned idx = t0; idx < items + ritems; idx += tstride) {
+        if (idx < items) {
+            const unsigned e = idx / L1;
+            const int d = (int)(idx - e * L1);
+            const int i = (int)(e / (unsigned)m), j = (int)(e - (unsigned)i * m);
+            const int ri = rinfo[i];
+            uint64_t v;
+            if (ri >= 0) {
+                v = d < L ? row_combo<SMALL>(src, a, piv, ri, rank, cap, m, j, d, src[(size_t)e * cap + d])
+                          : 0;
+            } else {
+                v = d >= 1 ? src[(size_t)e * cap + d - 1] : 0;
+                if (d == L && v != 0)
+                    atomicOr(&a.ctl[C_GROW], 1);
+            }
+            dst[(size_t)e * cap + d] = v;
+        } else {
+            const unsigned k = idx - items;
+            const unsigned e = k / span;
+            const int d = ord + 1 + (int)(k - e * span);
+            const int i = (int)(e / (unsigned)n), j = (int)(e - (unsigned)i * n);
+            const int ri = rinfo[i];
+            rd[(size_t)e * a.ordL + d] =
+                ri >= 0 ? row_combo<SMALL>(rs, a, piv, ri, rank, a.ordL, n, j, d, rs[(size_t)e * a.ordL + d])
+                        : rs[(size_t)e * a.ordL + d - 1];
+        }
     }
 }
 
@@ -820,7 +823,7 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     a.app[1] = ar.alloc((size_t)m * m * cap);
     a.R[0] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
     a.R[1] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
-    a.X = ar.alloc((size_t)m * m);
+    a.X = ar.alloc((size_t)m * m + kXPad);
     uint64_t *ints = ar.alloc((size_t)m + C_WORDS + 8);
     if (!a.app[0] || !a.app[1] || !a.R[0] || !a.R[1] || !a.X || !ints)
         return fail(PMLG_ERR_ALLOC, "pmbasis: device arena exhausted");
