@@ -31,7 +31,9 @@ struct CrtConst {
 
 __device__ __forceinline__ uint64_t mulmod_any(uint64_t a, uint64_t b, const ModConst &mc)
 {
-    return reduce128(mulhi64(a, b), a * b, mc);
+    uint64_t hi, lo;
+    mul128(a, b, hi, lo);
+    return reduce128(hi, lo, mc);
 }
 
 // out[e*out_stride + d] for d < width from the residues c_i[e*width + d] mod q_i

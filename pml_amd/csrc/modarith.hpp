@@ -127,11 +127,32 @@ PMLG_HD uint64_t reduce128(uint64_t hi, uint64_t lo, const ModConst &mc)
     return s >= mc.p ? s - mc.p : s;
 }
 
+// 64x64 -> 128 product (hi, lo).  On the device: four 32x32 -> 64
+// multiply-adds whose partial sums give both halves, a b = lo32(ll)
+// + 2^32 lo32(m2) + 2^64 (ah bh + hi32(m1) + hi32(m2)) with ll = al bl,
+// m1 = al bh + hi32(ll), m2 = ah bl + lo32(m1) (no partial sum overflows);
+// the separate high and low products take six.
+PMLG_HD void mul128(uint64_t a, uint64_t b, uint64_t &hi, uint64_t &lo)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32), bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
+    const uint64_t ll = (uint64_t)al * bl;
+    const uint64_t m1 = (uint64_t)al * bh + (ll >> 32);
+    const uint64_t m2 = (uint64_t)ah * bl + (uint32_t)m1;
+    hi = (uint64_t)ah * bh + (m1 >> 32) + (m2 >> 32);
+    lo = ((uint64_t)(uint32_t)m2 << 32) | (uint32_t)ll;
+#else
+    const unsigned __int128 r = (unsigned __int128)a * b;
+    hi = (uint64_t)(r >> 64);
+    lo = (uint64_t)r;
+#endif
+}
+
 // 64x64 -> 128 multiply-accumulate into (hi, lo)
 PMLG_HD void mac128(uint64_t &hi, uint64_t &lo, uint64_t a, uint64_t b)
 {
-    uint64_t l = a * b;
-    uint64_t h = mulhi64(a, b);
+    uint64_t h, l;
+    mul128(a, b, h, l);
     lo += l;
     hi += h + (lo < l ? 1 : 0);
 }

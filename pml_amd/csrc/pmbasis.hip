@@ -74,7 +74,9 @@ __device__ __forceinline__ uint64_t mulmod_g(uint64_t a, uint64_t b, const ModCo
 {
     if (mc.p < ((uint64_t)1 << 32))
         return barrett(a * b, mc.bm, mc.p);
-    return reduce128(mulhi64(a, b), a * b, mc);
+    uint64_t hi, lo;
+    mul128(a, b, hi, lo);
+    return reduce128(hi, lo, mc);
 }
 
 __device__ __forceinline__ uint64_t submod_g(uint64_t a, uint64_t b, uint64_t p)
@@ -153,18 +155,27 @@ struct Mont64Arith {
         const uint64_t u = hi + mulhi64(mm, p) + (lo != 0 ? 1 : 0);
         return u >= p ? u - p : u;
     }
-    __device__ V mul(V a, V b) const { return redc(mulhi64(a, b), a * b); }
+    __device__ V mul(V a, V b) const
+    {
+        uint64_t hi, lo;
+        mul128(a, b, hi, lo);
+        return redc(hi, lo);
+    }
     // Sums of up to five products, reduced once: t < 5 p^2 < 2^64 p, so the
     // REDC quotient is below (5p / 2^64 + 1) p < 2.25 p.
     struct Acc {
         uint64_t hi, lo;
     };
-    __device__ Acc prod(V a, V b) const { return Acc{mulhi64(a, b), a * b}; }
+    __device__ Acc prod(V a, V b) const
+    {
+        Acc t;
+        mul128(a, b, t.hi, t.lo);
+        return t;
+    }
     __device__ Acc fma(Acc t, V a, V b) const
     {
-        const uint64_t l = a * b, h = mulhi64(a, b);
-        const uint64_t lo = t.lo + l;
-        return Acc{t.hi + h + (lo < l ? 1 : 0), lo};
+        mac128(t.hi, t.lo, a, b);
+        return t;
     }
     __device__ V reduce(Acc t) const
     {
