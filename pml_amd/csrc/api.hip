@@ -440,6 +440,11 @@ void pmlg_ctx_destroy(pmlg_ctx ctx)
         ctx->tune_ws.release();
         ctx->crt_in.release();
         ctx->crt_out.release();
+        if (ctx->crt_fork)
+            (void)hipEventDestroy(ctx->crt_fork);
+        for (hipEvent_t e : ctx->crt_join)
+            if (e)
+                (void)hipEventDestroy(e);
         stagers_release(ctx);
         (void)hipStreamDestroy(ctx->stream);
     }

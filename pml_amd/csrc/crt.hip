@@ -115,33 +115,55 @@ pmlg_status crt_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
     const size_t outw = (size_t)(m * n * width);
     HIP_TRY(ctx->crt_out.ensure(3 * outw * 8), "hipMalloc(crt residues)");
     uint64_t *cres = static_cast<uint64_t *>(ctx->crt_out.ptr);
-    uint64_t *rA = nullptr, *rB = nullptr;
-    if (reduce) {
-        HIP_TRY(ctx->crt_in.ensure((size_t)(m * k * strideA + k * n * strideB) * 8 + 256),
-                "hipMalloc(crt inputs)");
-        rA = static_cast<uint64_t *>(ctx->crt_in.ptr);
-        rB = rA + m * k * strideA;
+    // one reduced copy of A and B per prime (the products run concurrently)
+    const size_t per = (size_t)(m * k * strideA + k * n * strideB);
+    if (reduce)
+        HIP_TRY(ctx->crt_in.ensure(3 * per * 8 + 256), "hipMalloc(crt inputs)");
+    // The three products are independent: they run on the sub-contexts'
+    // streams, forked from s by an event and joined back before the
+    // recombination, so each chain's launch gaps are filled by the others.
+    // A stream being captured keeps them in sequence on s.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+    const bool fork = cs == hipStreamCaptureStatusNone;
+    if (fork) {
+        if (!ctx->crt_fork) {
+            HIP_TRY(hipEventCreateWithFlags(&ctx->crt_fork, hipEventDisableTiming), "hipEventCreate");
+            for (hipEvent_t &e : ctx->crt_join)
+                HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        }
+        HIP_TRY(hipEventRecord(ctx->crt_fork, s), "hipEventRecord");
     }
     // the modulus of each product is q_i: the MFMA kernel is used where its
     // bound allows it, as for any other prime
     const int sub_method = method == PMLG_METHOD_PLAIN ? PMLG_METHOD_PLAIN : PMLG_METHOD_AUTO;
     for (int i = 0; i < 3; ++i) {
         pmlg_ctx q = ctx->crt[i];
+        hipStream_t si = fork ? q->stream : s;
+        if (fork)
+            HIP_TRY(hipStreamWaitEvent(si, ctx->crt_fork, 0), "hipStreamWaitEvent");
         const uint64_t *a = dA, *b = dB;
         if (reduce) {
-            hipLaunchKernelGGL(reduce_copy_kernel, dim3(crt_grid(m * k * la)), dim3(256), 0, s, rA, dA,
+            uint64_t *rA = static_cast<uint64_t *>(ctx->crt_in.ptr) + i * per;
+            uint64_t *rB = rA + m * k * strideA;
+            hipLaunchKernelGGL(reduce_copy_kernel, dim3(crt_grid(m * k * la)), dim3(256), 0, si, rA, dA,
                                m * k, strideA, la, q->p, q->mc.bm);
-            hipLaunchKernelGGL(reduce_copy_kernel, dim3(crt_grid(k * n * lb)), dim3(256), 0, s, rB, dB,
+            hipLaunchKernelGGL(reduce_copy_kernel, dim3(crt_grid(k * n * lb)), dim3(256), 0, si, rB, dB,
                                k * n, strideB, lb, q->p, q->mc.bm);
             HIP_TRY(hipGetLastError(), "reduce_copy_kernel");
             a = rA;
             b = rB;
         }
         pmlg_status st = mulmid_dev_strided(q, cres + i * outw, width, a, strideA, b, strideB, m, k,
-                                            n, la, lb, nlo, hi, sub_method, s);
+                                            n, la, lb, nlo, hi, sub_method, si);
         if (st != PMLG_OK)
             return st;
+        if (fork)
+            HIP_TRY(hipEventRecord(ctx->crt_join[i], si), "hipEventRecord");
     }
+    if (fork)
+        for (hipEvent_t e : ctx->crt_join)
+            HIP_TRY(hipStreamWaitEvent(s, e, 0), "hipStreamWaitEvent");
     // Garner constants
     const uint64_t q1 = kCrtPrimes[0], q2 = kCrtPrimes[1], q3 = kCrtPrimes[2], p = ctx->p;
     CrtConst c{};

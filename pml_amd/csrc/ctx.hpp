@@ -122,6 +122,9 @@ struct pmlg_ctx_s {
     bool is_prime = true;
     pmlg_ctx crt[3] = {nullptr, nullptr, nullptr};
     pmlg::DevBuf crt_in, crt_out;
+    // the three products run on the sub-contexts' streams, forked from and
+    // joined back to the caller's stream (events created on first use)
+    hipEvent_t crt_fork = nullptr, crt_join[3] = {nullptr, nullptr, nullptr};
     // pinned staging for entry-wise host transfers (created on first use)
     std::vector<pmlg::Stager> stagers;
 };
