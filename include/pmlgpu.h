@@ -129,7 +129,10 @@ pmlg_status pmlg_mbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift
                         const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
                         int64_t capP);
 /* PM-Basis with F and P in device memory (P capacity capP >= order + 1, its
- * length in *lenP is the computed one, not normalised); shift stays on the host. */
+ * length in *lenP is the computed one, not normalised: pmlg_pmbasis's basis
+ * followed by zero coefficients); shift stays on the host.  Work is enqueued on
+ * `stream` (NULL: the context's); the call reads one control word back per
+ * M-Basis leaf and returns with the stream idle. */
 pmlg_status pmlg_pmbasis_dev(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP,
                              int64_t *shift, const uint64_t *dF, int64_t m, int64_t n,
                              int64_t lenF, int64_t order, int method, void *stream);
