@@ -194,3 +194,17 @@ def test_kernel_rejects_bad_arguments(pml):
     with pytest.raises(pml.PmlgError) as ei:
         comp.kernel_basis(np.ones((3, 1, 2), dtype=np.uint64), np.zeros(3, dtype=np.int64))
     assert "prime" in str(ei.value)
+
+
+@pytest.mark.parametrize("m,n,ln,p,algo,kind", [
+    (100, 70, 3, P30, "zls", "uniform"),
+    (90, 66, 2, P60, "via_approx", "uniform"),
+    (128, 80, 2, P30, "zls", "uniform"),
+    (96, 72, 3, P60, "zls", "rkdef"),
+])
+def test_kernel_many_columns(pml, m, n, ln, p, algo, kind):
+    # more than 64 columns: the approximant bases inside hold two rows of the
+    # transposed residual per lane of the composing wave
+    rng = np.random.default_rng(m * 131 + n)
+    F = family(rng, kind, m, n, ln, p)
+    check(pml, p, F, np.zeros(m, dtype=np.int64), algo)
