@@ -294,17 +294,22 @@ __device__ __forceinline__ uint64_t lane_value(uint64_t v, int l)
 // pv * row_r - At[r][c] * row_pr with pv = At[pr][c].  The k-th row of the
 // reduced row echelon form is row prow[k] divided by its pivot.
 //
-// The columns are taken kPanel at a time.  One wave runs the pivot steps of
-// the panel on the panel's columns only and composes them: every row ends as
-// C[r][0] * (its own row at the panel's start) + sum_j C[r][j+1] * (the pivot
-// row of panel column j at the panel's start), the same exact residues as the
-// one-column-at-a-time sequence.  All threads then apply that rank-(1 + pivots) combination to the
-// whole matrix: two barriers per panel instead of one per column.
+// The columns are taken P at a time (a panel).  One wave runs the pivot
+// steps of the panel on the panel's columns only and composes them: every row
+// ends as C[r][0] * (its own row at the panel's start) + sum_j C[r][j+1] *
+// (the pivot row of panel column j at the panel's start), the same exact
+// residues as the one-column-at-a-time sequence.  The waves then apply that
+// combination to their own columns while the next panel is composed: one
+// barrier per panel instead of one per column.
 template <typename Arith, int H>
 __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, int ord)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     using V = typename Arith::V;
+    // panel width: 4 columns for 32-bit words, 2 for 64-bit ones, whose
+    // dearer products make a shorter composition pay for the extra barriers
+    constexpr int P = sizeof(V) == 8 ? 2 : kPanel;
+    static_assert(P <= kPanel, "LDS is sized for kPanel");
     const int m = a.m, n = a.n;
     const Arith ar(a.mc);
     const int ms = m | 1;                   // odd row stride of At
@@ -315,8 +320,8 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     int64_t *sh = reinterpret_cast<int64_t *>(smem);             // kMaxDim
     V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // npad x ms
     V *inv = At + npad * ms;                                     // kMaxDim
-    V *Ck = inv + kMaxDim;                                            // 2 x kMaxDim x (kPanel + 1)
-    int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxDim * (kPanel + 1));  // kMaxDim
+    V *Ck = inv + kMaxDim;                                            // 2 x kMaxDim x (P + 1)
+    int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxDim * (P + 1));  // kMaxDim
     int *pivcol = perm + kMaxDim;                                // kMaxDim
     int *plist = pivcol + kMaxDim;                               // kMaxDim
     int *prow = plist + kMaxDim;                                 // kMaxDim
@@ -397,7 +402,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     }
     __syncthreads();
 
-    // Gauss-Jordan by panels of kPanel columns.  Wave w owns the columns
+    // Gauss-Jordan by panels of P columns.  Wave w owns the columns
     // [w cw, (w + 1) cw) of At and applies every panel's combination to them;
     // the owner of the next panel then composes it from its own updated
     // columns (the LDS operations of one wave take effect in program order)
@@ -405,27 +410,31 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // the coefficients double-buffered.
     int rank = 0;
     const int wave = tid >> 6;
-    const int cw = m <= 16 * kPanel ? kPanel : 2 * kPanel;  // 16 waves cover kMaxDim columns
-    // compose the panel of columns c0 .. c0 + kPanel - 1 into buffer buf
+    // columns per wave: P times a power of two (so that cw divides the
+    // wave's 64 lanes), 16 waves covering m
+    int cw = P;
+    while (16 * cw < m)
+        cw *= 2;
+    // compose the panel of columns c0 .. c0 + P - 1 into buffer buf
     auto compose = [&](const int c0, const int buf) {
         __builtin_amdgcn_s_setprio(2);
         {
             // rows lane + 64 h (h < H) of the panel, their pivot flags and the
-            // coefficients C[.][0..kPanel] of the composition
-            V pn[H][kPanel], cc[H][kPanel + 1];
+            // coefficients C[.][0..P] of the composition
+            V pn[H][P], cc[H][P + 1];
             bool isp[H], ok[H];
-            int prs[kPanel] = {0};
+            int prs[P] = {0};
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 const int r = lane + 64 * h;
                 ok[h] = r < n;
                 isp[h] = ok[h] && ispiv[r];
 #pragma unroll
-                for (int j = 0; j < kPanel; ++j)
+                for (int j = 0; j < P; ++j)
                     pn[h][j] = ok[h] && c0 + j < m ? At[r * ms + c0 + j] : (V)0;
                 cc[h][0] = ar.one();
 #pragma unroll
-                for (int s2 = 1; s2 <= kPanel; ++s2)
+                for (int s2 = 1; s2 <= P; ++s2)
                     cc[h][s2] = 0;
             }
             int nb = 0, mask = 0;
@@ -456,9 +465,9 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 // broadcast the pivot row (panel values and coefficients): the
                 // pivot lane is uniform, so its registers are read into
                 // scalar registers
-                V qp[kPanel], qc[kPanel + 1];
+                V qp[P], qc[P + 1];
 #pragma unroll
-                for (int jj = j; jj < kPanel; ++jj)
+                for (int jj = j; jj < P; ++jj)
                     qp[jj] = lane_value(H == 2 && ph ? pn[H - 1][jj] : pn[0][jj], pl);
 #pragma unroll
                 for (int s2 = 1; s2 <= j + 1; ++s2)
@@ -475,7 +484,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                     const V fr = pn[h][j];
                     pn[h][j] = 0;
 #pragma unroll
-                    for (int jj = j + 1; jj < kPanel; ++jj)
+                    for (int jj = j + 1; jj < P; ++jj)
                         pn[h][jj] = ar.msub(pv, pn[h][jj], fr, qp[jj]);
                     cc[h][0] = ar.mul(pv, cc[h][0]);
 #pragma unroll
@@ -491,19 +500,22 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 }
                 ++nb;
             };
-            static_assert(kPanel == 4, "the panel steps below are written out for 4 columns");
+            static_assert(P >= 1 && P <= 4, "the panel steps below are written out for up to 4 columns");
             step(std::integral_constant<int, 0>{});
-            step(std::integral_constant<int, 1>{});
-            step(std::integral_constant<int, 2>{});
-            step(std::integral_constant<int, 3>{});
+            if constexpr (P > 1)
+                step(std::integral_constant<int, 1>{});
+            if constexpr (P > 2)
+                step(std::integral_constant<int, 2>{});
+            if constexpr (P > 3)
+                step(std::integral_constant<int, 3>{});
             // publish: coefficients, pivot flags, the pivot rows at the panel's start
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 const int r = lane + 64 * h;
                 if (ok[h]) {
 #pragma unroll
-                    for (int s2 = 0; s2 <= kPanel; ++s2)
-                        Ck[(buf * kMaxDim + r) * (kPanel + 1) + s2] = cc[h][s2];
+                    for (int s2 = 0; s2 <= P; ++s2)
+                        Ck[(buf * kMaxDim + r) * (P + 1) + s2] = cc[h][s2];
                     ispiv[r] = isp[h] ? 1 : 0;
                 }
             }
@@ -511,7 +523,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 s_nb[buf] = nb;
                 s_mask[buf] = mask;
 #pragma unroll
-                for (int k = 0; k < kPanel; ++k)
+                for (int k = 0; k < P; ++k)
                     s_prs[buf][k] = prs[k];
             }
         }
@@ -525,17 +537,17 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         const int cg = wave * cw + cl;
         if (cg >= m)
             return;
-        V pvr[kPanel];
+        V pvr[P];
 #pragma unroll
-        for (int q = 0; q < kPanel; ++q)
+        for (int q = 0; q < P; ++q)
             pvr[q] = mask & (1 << q) ? At[s_prs[buf][q] * ms + cg] : (V)0;
-        const V *C = Ck + buf * kMaxDim * (kPanel + 1);
+        const V *C = Ck + buf * kMaxDim * (P + 1);
         for (int r = g; r < n; r += G) {
-            const V *cr = C + r * (kPanel + 1);
+            const V *cr = C + r * (P + 1);
             V *x = At + r * ms + cg;
             typename Arith::Acc acc = ar.prod(cr[0], *x);
 #pragma unroll
-            for (int q = 0; q < kPanel; ++q)
+            for (int q = 0; q < P; ++q)
                 if (mask & (1 << q))
                     acc = ar.fma(acc, cr[q + 1], pvr[q]);
             *x = ar.reduce(acc);
@@ -550,7 +562,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         if (nb > 0)
             apply(buf, mask);
         rank += nb;
-        const int c1 = (k + 1) * kPanel;
+        const int c1 = (k + 1) * P;
         if (c1 >= m || rank >= n)
             break;
         if (wave == c1 / cw)
