@@ -118,7 +118,8 @@ pmlg_status pmlg_polmat_mulmid_entries(pmlg_ctx ctx, uint64_t *const *out, int64
  * at order `order` (same order for every column), by divide and conquer on the
  * order (nmod_poly_mat_pmbasis) or iteratively (nmod_poly_mat_mbasis).
  * P receives m x m entries of capacity capP >= order + 1 (entry (i,j)
- * coefficient d at P[(i*m + j)*capP + d]); *lenP its normalised length;
+ * coefficient d at P[(i*m + j)*capP + d]; every coefficient up to capP is
+ * written, zeros past the basis); *lenP its normalised length;
  * shift (length m) is the input shift on entry and the shifted row degree of
  * P on return.  1 <= m, n <= 128.  The products inside PM-Basis run on the
  * device with the given pointwise method. */

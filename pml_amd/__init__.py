@@ -292,7 +292,7 @@ class Context:
         m, n, lf = F.shape
         sh = np.ascontiguousarray(shift, dtype=np.int64).copy()
         cap = max(order, 0) + 1
-        P = np.zeros((m, m, cap), dtype=np.uint64)
+        P = np.empty((m, m, cap), dtype=np.uint64)  # every coefficient is written
         L = ctypes.c_int64(0)
         _check(fn(self.handle, _p(P), ctypes.byref(L), sh.ctypes.data_as(_i64p), _p(F), m, n,
                   lf, order, cap, *extra))

@@ -170,11 +170,13 @@ pmlg_status const_rank_dev(pmlg_ctx ctx, uint64_t *dM, uint64_t *dScratch, int64
 // Upload: device entry e at dst + e*width <- ptr[e] (len[e] words, clipped and
 // zero padded); returns when complete.  Download: ptr[e] <- device entry e
 // of width words, normalised (lenOut[e] = length without trailing zeros),
-// after the work enqueued on `after`.
+// after the work enqueued on `after`; with zero_to > 0 each ptr[e] is also
+// zero-filled from its length up to zero_to words.
 pmlg_status stage_upload(pmlg_ctx ctx, uint64_t *dst, const uint64_t *const *ptr,
                          const int64_t *len, int64_t count, int64_t width);
 pmlg_status stage_download(pmlg_ctx ctx, uint64_t *const *ptr, int64_t *lenOut,
-                           const uint64_t *src, int64_t count, int64_t width, hipStream_t after);
+                           const uint64_t *src, int64_t count, int64_t width, hipStream_t after,
+                           int64_t zero_to = 0);
 void stagers_release(pmlg_ctx ctx);
 
 }  // namespace pmlg

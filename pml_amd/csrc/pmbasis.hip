@@ -1011,9 +1011,9 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
     HIP_TRY(hipMemcpyAsync(shift, dShift, (size_t)m * 8, hipMemcpyDeviceToHost, s), "D2H(shift)");
     const int64_t L = std::min<int64_t>(P.len, capP);
     if (hP) {
-        // entries packed to the computed length on the device, then read back
-        const uint64_t *src = P.d;
-        if (P.cap != L && L > 0) {
+        // entries packed to the computed length on the device (stride L)
+        const uint64_t *src = P.cap == L ? P.d : nullptr;
+        if (!src) {
             uint64_t *packed = ar.alloc((size_t)(m * m * L));
             if (packed) {
                 hipLaunchKernelGGL(copy_poly_kernel, dim3(grid_for(m * m * L)), dim3(256), 0, s, packed,
@@ -1022,21 +1022,36 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
                 src = packed;
             }
         }
-        const int64_t scap = src == P.d ? P.cap : L;
-        std::vector<uint64_t> tmp((size_t)(m * m * scap));
-        HIP_TRY(hipMemcpyAsync(tmp.data(), src, tmp.size() * 8, hipMemcpyDeviceToHost, s), "D2H(P)");
-        HIP_TRY(hipStreamSynchronize(s), "sync");
-        // normalised length (the basis has degree at most order)
         int64_t Ln = 0;
-        for (int64_t e = 0; e < m * m; ++e)
-            for (int64_t d = L - 1; d >= Ln; --d)
-                if (tmp[(size_t)(e * scap + d)]) {
-                    Ln = d + 1;
-                    break;
-                }
-        for (int64_t e = 0; e < m * m; ++e) {
-            std::memcpy(hP + e * capP, tmp.data() + e * scap, (size_t)Ln * 8);
-            std::memset(hP + e * capP + Ln, 0, (size_t)(capP - Ln) * 8);
+        if (src) {
+            // through the pinned staging chunks: each entry trimmed of its zero
+            // top coefficients and zero-filled up to capP; the normalised
+            // length is the longest entry's (the basis has degree at most order)
+            std::vector<uint64_t *> ptr((size_t)(m * m));
+            std::vector<int64_t> len((size_t)(m * m));
+            for (int64_t e = 0; e < m * m; ++e)
+                ptr[(size_t)e] = hP + e * capP;
+            pmlg_status st2 = stage_download(ctx, ptr.data(), len.data(), src, m * m, L, s, capP);
+            if (st2 != PMLG_OK)
+                return st2;
+            HIP_TRY(hipStreamSynchronize(s), "sync");  // the shifts' read-back
+            for (int64_t e = 0; e < m * m; ++e)
+                Ln = std::max(Ln, len[(size_t)e]);
+        } else {
+            // no arena room to pack: one transfer of the entries at capacity
+            std::vector<uint64_t> tmp((size_t)(m * m * P.cap));
+            HIP_TRY(hipMemcpyAsync(tmp.data(), P.d, tmp.size() * 8, hipMemcpyDeviceToHost, s), "D2H(P)");
+            HIP_TRY(hipStreamSynchronize(s), "sync");
+            for (int64_t e = 0; e < m * m; ++e)
+                for (int64_t d = L - 1; d >= Ln; --d)
+                    if (tmp[(size_t)(e * P.cap + d)]) {
+                        Ln = d + 1;
+                        break;
+                    }
+            for (int64_t e = 0; e < m * m; ++e) {
+                std::memcpy(hP + e * capP, tmp.data() + e * P.cap, (size_t)Ln * 8);
+                std::memset(hP + e * capP + Ln, 0, (size_t)(capP - Ln) * 8);
+            }
         }
         *lenP = Ln;
     } else {
@@ -1083,8 +1098,17 @@ static pmlg_status basis_host(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t 
     hipStream_t s = ctx->stream;
     const size_t fb = (size_t)(m * n * (lenF > 0 ? lenF : 1)) * 8;
     HIP_TRY(ctx->in_a.ensure(fb), "hipMalloc(F)");
-    if (lenF > 0)
-        HIP_TRY(hipMemcpyAsync(ctx->in_a.ptr, F, fb, hipMemcpyHostToDevice, s), "H2D(F)");
+    if (lenF > 0) {
+        // through the pinned staging chunks, entry by entry
+        std::vector<const uint64_t *> ptr((size_t)(m * n));
+        std::vector<int64_t> len((size_t)(m * n), lenF);
+        for (int64_t e = 0; e < m * n; ++e)
+            ptr[(size_t)e] = F + e * lenF;
+        st = stage_upload(ctx, static_cast<uint64_t *>(ctx->in_a.ptr), ptr.data(), len.data(), m * n,
+                          lenF);
+        if (st != PMLG_OK)
+            return st;
+    }
     return basis_common(ctx, nullptr, capP, lenP, shift, static_cast<const uint64_t *>(ctx->in_a.ptr),
                         lenF, m, n, lenF, order, method, iterative, s, P);
 }

@@ -164,13 +164,17 @@ pmlg_status stage_upload(pmlg_ctx ctx, uint64_t *dst, const uint64_t *const *ptr
 // normalised: lenOut[e] is its length without trailing zeros and only those
 // words are written.  The copies wait for work already enqueued on `after`.
 pmlg_status stage_download(pmlg_ctx ctx, uint64_t *const *ptr, int64_t *lenOut,
-                           const uint64_t *src, int64_t count, int64_t width, hipStream_t after)
+                           const uint64_t *src, int64_t count, int64_t width, hipStream_t after,
+                           int64_t zero_to)
 {
     if (count <= 0)
         return PMLG_OK;
     if (width <= 0) {
-        for (int64_t e = 0; e < count; ++e)
+        for (int64_t e = 0; e < count; ++e) {
             lenOut[e] = 0;
+            if (zero_to > 0)
+                std::memset(ptr[e], 0, (size_t)zero_to * 8);
+        }
         return PMLG_OK;
     }
     const size_t cb = chunk_bytes(width);
@@ -203,6 +207,8 @@ pmlg_status stage_download(pmlg_ctx ctx, uint64_t *const *ptr, int64_t *lenOut,
                     --l;
                 if (l)
                     std::memcpy(ptr[e], s, (size_t)l * 8);
+                if (zero_to > l)
+                    std::memset(ptr[e] + l, 0, (size_t)(zero_to - l) * 8);
                 lenOut[e] = l;
             }
             return PMLG_OK;
