@@ -515,10 +515,19 @@ pmlg_status pmlg_polmat_mulmid(pmlg_ctx ctx, uint64_t *out, const uint64_t *A, c
     HIP_TRY(ctx->in_a.ensure(ba ? ba : 8), "hipMalloc(A)");
     HIP_TRY(ctx->in_b.ensure(bb ? bb : 8), "hipMalloc(B)");
     HIP_TRY(ctx->out_c.ensure(bc), "hipMalloc(C)");
-    if (ba)
-        HIP_TRY(hipMemcpyAsync(ctx->in_a.ptr, A, ba, hipMemcpyHostToDevice, s), "H2D(A)");
-    if (bb)
-        HIP_TRY(hipMemcpyAsync(ctx->in_b.ptr, B, bb, hipMemcpyHostToDevice, s), "H2D(B)");
+    // operands and result move through the pinned staging chunks (packed
+    // host arrays seen as entries of one length each)
+    auto upload = [&](void *dst, const uint64_t *src, int64_t count, int64_t len) -> pmlg_status {
+        std::vector<const uint64_t *> ptr((size_t)count);
+        std::vector<int64_t> lens((size_t)count, len);
+        for (int64_t e = 0; e < count; ++e)
+            ptr[(size_t)e] = src + e * len;
+        return stage_upload(ctx, static_cast<uint64_t *>(dst), ptr.data(), lens.data(), count, len);
+    };
+    if (ba && (st = upload(ctx->in_a.ptr, A, m * k, lenA)) != PMLG_OK)
+        return st;
+    if (bb && (st = upload(ctx->in_b.ptr, B, k * n, lenB)) != PMLG_OK)
+        return st;
     st = mulmid_dev_locked(ctx, static_cast<uint64_t *>(ctx->out_c.ptr),
                            static_cast<const uint64_t *>(ctx->in_a.ptr),
                            static_cast<const uint64_t *>(ctx->in_b.ptr), m, k, n, lenA, lenB, nlo,
@@ -527,7 +536,17 @@ pmlg_status pmlg_polmat_mulmid(pmlg_ctx ctx, uint64_t *out, const uint64_t *A, c
         (void)hipStreamSynchronize(s);
         return st;
     }
-    HIP_TRY(hipMemcpyAsync(out, ctx->out_c.ptr, bc, hipMemcpyDeviceToHost, s), "D2H(C)");
+    // every output word is written: each entry's zero top is filled
+    std::vector<uint64_t *> po((size_t)(m * n));
+    std::vector<int64_t> lo((size_t)(m * n));
+    for (int64_t e = 0; e < m * n; ++e)
+        po[(size_t)e] = out + e * width;
+    st = stage_download(ctx, po.data(), lo.data(), static_cast<const uint64_t *>(ctx->out_c.ptr), m * n,
+                        width, s, width);
+    if (st != PMLG_OK) {
+        (void)hipStreamSynchronize(s);
+        return st;
+    }
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     return PMLG_OK;
 }
