@@ -333,7 +333,14 @@ class Context:
         sh0 = np.ascontiguousarray(shift, dtype=np.int64).copy()
         a = {"zls": KERNEL_ZLS, "via_approx": KERNEL_VIA_APPROX}[algo] if isinstance(algo, str) \
             else int(algo)
-        cap = int(cap) if cap else (min(m, n) + 1) * max(lf, 1) + int(sh0.max() - sh0.min()) + 1
+        if not cap:
+            # first guess from the generic kernel degree n d / (m - n) (the
+            # shifted row degrees of a kernel basis add up to at most n d),
+            # with room to spare; the call reports the exact length when the
+            # guess is short and is then repeated with it
+            spread = int(sh0.max() - sh0.min()) if m else 0
+            cap = 2 * (n * max(lf, 1) // max(m - n, 1)) + spread + 64
+        cap = int(cap)
         while True:
             sh = sh0.copy()
             piv = np.zeros(m, dtype=np.int64)

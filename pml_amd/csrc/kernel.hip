@@ -638,12 +638,18 @@ pmlg_status pmlg_kernel_basis(pmlg_ctx ctx, uint64_t *K, int64_t capK, int64_t *
         return fail(PMLG_ERR_ARG, "kernel basis: output capacity " + std::to_string(capK) +
                                       " below the basis length " + std::to_string(L));
     if (nul > 0) {
-        // rows [0, nul) with capacity capK, then one copy to the host
+        // rows [0, nul) packed to the basis length on the device, then read
+        // back through the pinned staging chunks, whose unpacking threads
+        // zero-fill each entry up to capK
+        const int64_t Lw = L > 0 ? L : 1;
         DMat out;
-        KTRY(alloc(out, nul, m, capK));
-        KTRY(gather(X, out, capK, Kd, 0, m, iota_vec(nul), iota_vec(nul)));
-        HIP_TRY(hipMemcpyAsync(K, out.d, (size_t)(nul * m * capK) * 8, hipMemcpyDeviceToHost, X.s),
-                "D2H(K)");
+        KTRY(alloc(out, nul, m, Lw));
+        KTRY(gather(X, out, Lw, Kd, 0, m, iota_vec(nul), iota_vec(nul)));
+        std::vector<uint64_t *> ptr((size_t)(nul * m));
+        std::vector<int64_t> len((size_t)(nul * m));
+        for (int64_t e = 0; e < nul * m; ++e)
+            ptr[(size_t)e] = K + e * capK;
+        KTRY(stage_download(ctx, ptr.data(), len.data(), out.d, nul * m, Lw, X.s, capK));
     }
     HIP_TRY(hipStreamSynchronize(X.s), "sync");
     return PMLG_OK;
