@@ -623,9 +623,13 @@ pmlg_status pmlg_kernel_basis(pmlg_ctx ctx, uint64_t *K, int64_t capK, int64_t *
     Kx X{ctx, ctx->stream, method, {}};
     DMat dF;
     if (lenF > 0) {
+        // through the pinned staging chunks, entry by entry
         KTRY(alloc(dF, m, n, lenF));
-        HIP_TRY(hipMemcpyAsync(dF.d, F, (size_t)(m * n * lenF) * 8, hipMemcpyHostToDevice, X.s),
-                "H2D(F)");
+        std::vector<const uint64_t *> ptr((size_t)(m * n));
+        std::vector<int64_t> len((size_t)(m * n), lenF);
+        for (int64_t e = 0; e < m * n; ++e)
+            ptr[(size_t)e] = F + e * lenF;
+        KTRY(stage_upload(ctx, dF.d, ptr.data(), len.data(), m * n, lenF));
     }
     int64_t nul = 0;
     DMat Kd;
