@@ -39,7 +39,7 @@ namespace {
 
 constexpr int kStepThreads = 1024;
 constexpr int kMaxDim = 128;  // m, n limits of the device M-Basis step
-constexpr int kPanel = 4;     // Gauss-Jordan columns composed per pair of barriers
+constexpr int kPanel = 4;     // widest Gauss-Jordan panel (LDS is sized for it)
 constexpr int kXPad = 32;     // words after X read (and discarded) by the last update group
 
 // control words of one leaf
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             }
             int nb = 0, mask = 0;
             // one pivot step on panel column j (a compile-time index: the
-            // convergent ballots and shuffles keep the compiler from unrolling
+            // convergent ballots and lane reads keep the compiler from unrolling
             // a loop around them, and the arrays must stay in registers)
             auto step = [&](auto jc) {
                 constexpr int j = decltype(jc)::value;
@@ -508,7 +508,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 step(std::integral_constant<int, 2>{});
             if constexpr (P > 3)
                 step(std::integral_constant<int, 3>{});
-            // publish: coefficients, pivot flags, the pivot rows at the panel's start
+            // publish: coefficients, pivot flags, pivot rows
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 const int r = lane + 64 * h;
