@@ -283,8 +283,9 @@ __device__ __forceinline__ void load_tw(const W *__restrict__ tab, int idx, W &w
 }
 
 // Butterfly constants.  Strict mode keeps values in [0, 2p) (any p < 2^62);
-// loose mode (64-bit words, p < 2^61) keeps them in [0, 4p) and uses a
-// cheaper Shoup product whose quotient may be short by at most 2.
+// loose mode (64-bit words, p < 2^61) keeps them in [0, 4p) in the forward
+// transform and [0, 8p) in the inverse, and uses a cheaper Shoup product
+// whose quotient may be short by at most 2.
 template <typename W>
 struct Bfly {
     W p, p2, p4, np, np4;  // np = -p mod 2^64, np4 = -4p mod 2^64
@@ -355,21 +356,39 @@ __device__ __forceinline__ void bfly_dif1(W &a, W &c, const Bfly<W> &k)
     }
 }
 
+// Inverse (DIT) butterflies.  Loose mode keeps the inverse's values in
+// [0, 8p) (8p < 2^64 for p < 2^61): the upper input a is brought to [0, 4p)
+// once, the product v is in [0, 4p) for any 64-bit c, so a + v and a - v + 4p
+// are back in [0, 8p) with one conditional step per butterfly.
 template <typename W, bool LOOSE>
 __device__ __forceinline__ void bfly_dit(W &a, W &c, W w, W wq, const Bfly<W> &k)
 {
     if constexpr (LOOSE) {
+        const W ar = a >= k.p4 ? a + k.np4 : a;
         const W v = shoup4(c, w, wq, k.np);
-        W d = sub_4p(a, v, k.p4);
-        d = d >= k.p4 ? d + k.np4 : d;
-        a = add_4p(a, v, k.p4, k.np4);
-        c = d;
+        a = ar + v;
+        c = sub_4p(ar, v, k.p4);
     } else {
         const W u = a;
         const W v = shoup_lazy(c, w, wq, k.p);
         a = add_2p<W>(u, v, k.p2);
         const W d = u + k.p2 - v;
         c = d >= k.p2 ? d - k.p2 : d;
+    }
+}
+
+// inverse butterfly whose twiddle is 1: loose inputs in [0, 8p) are brought
+// to [0, 4p) first, so (a + c, a - c + 4p) stay in [0, 8p)
+template <typename W, bool LOOSE>
+__device__ __forceinline__ void bfly_dit1(W &a, W &c, const Bfly<W> &k)
+{
+    if constexpr (LOOSE) {
+        const W ar = a >= k.p4 ? a + k.np4 : a;
+        const W cr = c >= k.p4 ? c + k.np4 : c;
+        a = ar + cr;
+        c = sub_4p(ar, cr, k.p4);
+    } else {
+        bfly_dif1<W, false>(a, c, k);
     }
 }
 
@@ -380,6 +399,15 @@ __device__ __forceinline__ W bfly_final(W x, const Bfly<W> &k)
     if constexpr (LOOSE)
         x = x >= k.p2 ? x - k.p2 : x;
     return x >= k.p ? x - k.p : x;
+}
+
+// the inverse's [0, 8p) (loose) or [0, 2p) (strict) -> [0, p)
+template <typename W, bool LOOSE>
+__device__ __forceinline__ W bfly_final_inv(W x, const Bfly<W> &k)
+{
+    if constexpr (LOOSE)
+        x = x >= k.p4 ? x + k.np4 : x;
+    return bfly_final<W, LOOSE>(x, k);
 }
 
 template <int L, int LOGE, int S0>
@@ -483,7 +511,7 @@ __device__ __forceinline__ void inv_stages(W (&x)[1 << LOGE], const W (&tw)[2 * 
             for (int jh = 0; jh < (1 << (R - 1 - LB)); ++jh) {
                 const int j = (jh << (LB + 1)) | jl;
                 if (PS::BLO == 0 && jl == 0)  // twiddle w^0 = 1 (see fwd_stages)
-                    bfly_dif1<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], k);
+                    bfly_dit1<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], k);
                 else
                     bfly_dit<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
             }
@@ -726,7 +754,7 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
         W *o = tmp + e * N + (int64_t)gg * M;
 #pragma unroll
         for (int j = 0; j < E; ++j)
-            o[(j << (L - LOGE)) | t] = bfly_final<W, LOOSE>(x[j], k);
+            o[(j << (L - LOGE)) | t] = bfly_final_inv<W, LOOSE>(x[j], k);
     }
 }
 
