@@ -62,6 +62,7 @@ struct LeafArgs {
     int cap;      // coefficient capacity of appbas (ordL + 1)
     uint64_t *app[2];  // m x m x cap, entry-major
     uint64_t *R[2];    // m x n x ordL, entry-major
+    uint64_t *Rn;      // m x n: the coming step's residual coefficient
     uint64_t *X;       // nullity x rank, row stride m
     int *piv;          // m
     int *rowinfo;      // m: t >= 0 nullity row t, -1-r rank row r
@@ -199,6 +200,7 @@ struct Mont64Arith {
     __device__ V one() const { return r1; }
     __device__ V to(uint64_t x) const { return mul(x, r2); }
     __device__ uint64_t from(V x) const { return redc(0, x); }
+    static __device__ V load(const uint64_t *w) { return *w; }
     // a = x R: the plain inverse of a is x^-1 R^-1, and x^-1 R = mul(mul(a^-1, R^2), R^2)
     __device__ V inv(V a) const { return mul(mul(inv_euclid(a, p), r2), r2); }
 };
@@ -251,6 +253,8 @@ struct Mont32Arith {
     }
     __device__ V one() const { return r1; }
     __device__ V to(uint64_t x) const { return mul((V)x, r2); }
+    // a residue below p < 2^30 is the low half of its 64-bit word
+    static __device__ V load(const uint64_t *w) { return *reinterpret_cast<const uint32_t *>(w); }
     __device__ uint64_t from(V x) const { return redc(x); }
     __device__ V inv(V a) const
     {
@@ -313,13 +317,10 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     const int m = a.m, n = a.n;
     const Arith ar(a.mc);
     const int ms = m | 1;                   // odd row stride of At
-    const int rstep = kStepThreads / m;     // m <= kMaxDim <= kStepThreads
-    const int rpt = (n + rstep - 1) / rstep;  // rows of At per thread
-    const int npad = rpt * rstep;           // rows n .. npad-1 are zero
     // 64-bit words first (sh), then the V arrays, then the int arrays
     int64_t *sh = reinterpret_cast<int64_t *>(smem);             // kMaxDim
-    V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // npad x ms
-    V *inv = At + npad * ms;                                     // kMaxDim
+    V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // n x ms
+    V *inv = At + n * ms;                                        // kMaxDim
     V *Ck = inv + kMaxDim;                                            // 2 x kMaxDim x (P + 1)
     int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxDim * (P + 1));  // kMaxDim
     int *pivcol = perm + kMaxDim;                                // kMaxDim
@@ -333,10 +334,13 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     const int lane = tid & 63;
     const uint64_t p = a.mc.p;
 
-    // Shifts and control words first, the shifts stored before the residual
-    // loads are issued (a wait on a vector load also waits on those issued
-    // before it); every thread reads the control words, so the residual loads
-    // start at once, and thread 0 writes them back only after full barriers.
+    // Shifts, control words and this step's residual coefficient.  The
+    // coefficient sits in the fixed buffer Rn (m x n, row-major), written by
+    // the previous update or the leaf's initialisation, so its loads wait for
+    // no control word and are contiguous across the workgroup; the shifts are
+    // stored to LDS before those loads are issued (a wait on a vector load
+    // also waits on those issued before it).  Thread 0 writes the control
+    // words back only after full barriers.
     const int64_t shv = tid < m ? a.shift[tid] : 0;  // m <= kMaxDim <= kStepThreads
     int *c = a.ctl;
     const int flip = c[C_FLIP], cur0 = c[C_CUR], done = c[C_DONE];
@@ -346,21 +350,17 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         colpiv[tid] = 0;
     }
     const int cur = flip ? cur0 ^ 1 : cur0;
-    const uint64_t *R = a.R[cur];
-    // this thread's elements: row col of the (unpermuted) residual, columns
-    // r0 + k rstep (k < rpt), stored to column pos[col] of At once the
-    // permutation is known; the first kPre are loaded before the sort
-    const int col = tid % m;
-    const int r0 = tid / m;
-    const bool active = r0 < rstep;
+    // element q = tid + k kStepThreads of the coefficient (row q / n of the
+    // unpermuted residual, column q % n), stored to At[q % n][pos[q / n]] once
+    // the permutation is known; the first kPre per thread are loaded before
+    // the sort (indices past m n clamped, their values unused)
+    const int mn = m * n;
     constexpr int kPre = 4;
-    uint64_t pre[kPre];
-    if (!done && active) {
+    V pre[kPre];
 #pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-            const int r = r0 + k * rstep;
-            pre[k] = k < rpt && r < n ? R[((int64_t)col * n + r) * a.ordL + ord] : 0;
-        }
+    for (int k = 0; k < kPre; ++k) {
+        const int q = tid + k * kStepThreads;
+        pre[k] = Arith::load(a.Rn + (q < mn ? q : mn - 1));
     }
     for (int r = tid; r < kMaxDim; r += kStepThreads)
         ispiv[r] = 0;
@@ -388,17 +388,17 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     lds_barrier();
     for (int i = tid; i < m; i += kStepThreads)
         perm[pos[i]] = i;
-    if (active) {
-        // At has npad rows, rows n .. npad-1 zero
-        const int pc = pos[col];
 #pragma unroll
-        for (int k = 0; k < kPre; ++k)
-            if (k < rpt)
-                At[(r0 + k * rstep) * ms + pc] = ar.to(pre[k]);
-        for (int k = kPre; k < rpt; ++k) {
-            const int r = r0 + k * rstep;
-            At[r * ms + pc] = r < n ? ar.to(R[((int64_t)col * n + r) * a.ordL + ord]) : (V)0;
+    for (int k = 0; k < kPre; ++k) {
+        const int q = tid + k * kStepThreads;
+        if (q < mn) {
+            const int i = q / n, j = q - i * n;
+            At[j * ms + pos[i]] = ar.to(pre[k]);
         }
+    }
+    for (int q = tid + kPre * kStepThreads; q < mn; q += kStepThreads) {
+        const int i = q / n, j = q - i * n;
+        At[j * ms + pos[i]] = ar.to(Arith::load(a.Rn + q));
     }
     __syncthreads();
 
@@ -694,14 +694,22 @@ template <bool SMALL>
 __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
 {
     const int mode = a.ctl[C_MODE];
-    if (mode == 0)
-        return;
     const int cur = a.ctl[C_CUR];
+    const unsigned tstride = gridDim.x * blockDim.x;
+    const unsigned t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 0) {
+        // residual unchanged: the coming step's coefficient is ord + 1
+        // (nothing to do once the basis is complete)
+        if (ord + 1 < a.ordL && !a.ctl[C_DONE]) {
+            const uint64_t *rs = a.R[cur];
+            for (unsigned e = t0; e < (unsigned)(a.m * a.n); e += tstride)
+                a.Rn[e] = rs[(size_t)e * a.ordL + ord + 1];
+        }
+        return;
+    }
     const int m = a.m, n = a.n, cap = a.cap;
     const uint64_t *src = a.app[cur];
     uint64_t *dst = a.app[cur ^ 1];
-    const unsigned tstride = gridDim.x * blockDim.x;
-    const unsigned t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (mode == 2) {
         const int s = a.ctl[C_SHIFT], Lold = a.ctl[C_LOLD];
         const unsigned Lnew = (unsigned)(Lold + s);
@@ -753,9 +761,12 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
             const int d = ord + 1 + (int)(k - e * span);
             const int i = (int)(e / (unsigned)n), j = (int)(e - (unsigned)i * n);
             const int ri = rinfo[i];
-            rd[(size_t)e * a.ordL + d] =
+            const uint64_t v =
                 ri >= 0 ? row_combo<SMALL>(rs, a, piv, ri, rank, a.ordL, n, j, d, rs[(size_t)e * a.ordL + d])
                         : rs[(size_t)e * a.ordL + d - 1];
+            rd[(size_t)e * a.ordL + d] = v;
+            if (d == ord + 1)
+                a.Rn[e] = v;  // the coming step's coefficient
         }
     }
 }
@@ -778,7 +789,10 @@ __global__ void leaf_init_kernel(LeafArgs a, const uint64_t *__restrict__ F, int
     for (int64_t idx = t0; idx < itemsR; idx += tstride) {
         const int64_t e = idx / a.ordL;
         const int d = (int)(idx - e * a.ordL);
-        a.R[0][idx] = d < lenF ? F[e * strideF + d] : 0;
+        const uint64_t v = d < lenF ? F[e * strideF + d] : 0;
+        a.R[0][idx] = v;
+        if (d == 0)
+            a.Rn[e] = v;
     }
     if (t0 < C_WORDS)
         a.ctl[t0] = (t0 == C_LEN) ? 1 : 0;
@@ -846,9 +860,10 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     a.app[1] = ar.alloc((size_t)m * m * cap);
     a.R[0] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
     a.R[1] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
+    a.Rn = ar.alloc((size_t)m * n);
     a.X = ar.alloc((size_t)m * m + kXPad);
     uint64_t *ints = ar.alloc((size_t)m + C_WORDS + 8);
-    if (!a.app[0] || !a.app[1] || !a.R[0] || !a.R[1] || !a.X || !ints)
+    if (!a.app[0] || !a.app[1] || !a.R[0] || !a.R[1] || !a.Rn || !a.X || !ints)
         return fail(PMLG_ERR_ALLOC, "pmbasis: device arena exhausted");
     a.piv = reinterpret_cast<int *>(ints);
     a.rowinfo = a.piv + m;
@@ -859,15 +874,12 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     hipLaunchKernelGGL(leaf_init_kernel, dim3(grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL)),
                        dim3(256), 0, s, a, F, strideF, lt);
     HIP_TRY(hipGetLastError(), "leaf_init_kernel");
-    // LDS of the step kernel: shifts, At padded to a multiple of the row step,
+    // LDS of the step kernel: shifts, At (n rows of odd stride m | 1),
     // inverses, two coefficient buffers, then 7 int arrays (sizes as in the kernel)
-    const int rstep = kStepThreads / m;
-    const size_t npad = (size_t)((n + rstep - 1) / rstep) * rstep;
-    const size_t lds = (npad * (m | 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 + 7 * kMaxDim * sizeof(int);
+    const size_t lds = ((size_t)n * (m | 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 + 7 * kMaxDim * sizeof(int);
     static bool attr = false;
     if (!attr) {
-        const size_t npmax = kMaxDim + kStepThreads / kMaxDim - 1;
-        const int maxlds = (int)((npmax * (kMaxDim + 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 +
+        const int maxlds = (int)(((size_t)kMaxDim * (kMaxDim + 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 +
                                  7 * kMaxDim * sizeof(int));
         const void *kernels[] = {(const void *)mbasis_step_kernel<Mont64Arith, 1>,
                                  (const void *)mbasis_step_kernel<Mont64Arith, 2>,
