@@ -122,7 +122,9 @@ pmlg_status pmlg_polmat_mulmid_entries(pmlg_ctx ctx, uint64_t *const *out, int64
  * written, zeros past the basis); *lenP its normalised length;
  * shift (length m) is the input shift on entry and the shifted row degree of
  * P on return.  1 <= m, n <= 128.  The products inside PM-Basis run on the
- * device with the given pointwise method. */
+ * device with the given pointwise method.  pmlg_mbasis returns
+ * PMLG_ERR_UNSUPPORTED when m^2 (order + 1) + m n order >= 2^31 (PM-Basis
+ * has no such limit: its M-Basis leaves have order at most 32). */
 pmlg_status pmlg_pmbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
                          const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
                          int64_t capP, int method);

@@ -1000,8 +1000,12 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
         return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
     if (capP < order + 1)
         return fail(PMLG_ERR_ARG, "approximant basis: output capacity must be at least order + 1");
-    // device arena
+    // the M-Basis kernels index one leaf's items in 32 bits
     const int64_t ordleaf = iterative ? order : std::min<int64_t>(order, kPmbasisThres);
+    if (m * m * (ordleaf + 1) + m * n * ordleaf >= ((int64_t)1 << 31))
+        return fail(PMLG_ERR_UNSUPPORTED, "M-Basis: m^2 (order + 1) + m n order must stay below 2^31 "
+                                          "(use PM-Basis for this order)");
+    // device arena
     size_t need = arena_bytes(m, n, order) +
                   (size_t)(m * m * (ordleaf + 2) * 2 + m * n * (ordleaf + 1) * 2) * 8;
     HIP_TRY(ctx->arena.ensure(need), "hipMalloc(arena)");
