@@ -121,8 +121,8 @@ def stage_bytes(batch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: how this path would shard over several GPUs "
-              "is described in DESIGN.md but not run (--gpus %d requested)" % args.gpus,
+        print("bench.py measures one GPU only: sharding this path over several GPUs "
+              "is not built (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import pml_amd

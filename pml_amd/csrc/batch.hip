@@ -162,7 +162,8 @@ bool config_valid(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, i
     return true;
 }
 
-// Threshold rule (the fallback, and the judge of the learned choice):
+// Threshold rule (the fallback, and the baseline the learned choice is
+// measured against):
 // direct convolution when both operands are short, else transforms with the
 // MFMA pointwise kernel whenever the shape and modulus allow it.
 int threshold_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB)
