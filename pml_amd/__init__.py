@@ -135,6 +135,10 @@ def lib():
                                        c_int]
             L.pmlg_mbasis.restype = st
             L.pmlg_mbasis.argtypes = [vp, _u64p, _i64p, _i64p, _u64p, i64, i64, i64, i64, i64]
+            L.pmlg_pmbasis_set_leaf_order.restype = st
+            L.pmlg_pmbasis_set_leaf_order.argtypes = [vp, i64]
+            L.pmlg_pmbasis_leaf_order.restype = st
+            L.pmlg_pmbasis_leaf_order.argtypes = [vp, i64, i64, c_int, _i64p]
         L.pmlg_dispatch_set_refine.restype = None
         L.pmlg_dispatch_set_refine.argtypes = [c_int]
         if hasattr(L, "pmlg_polmat_mul_entries"):
@@ -318,6 +322,17 @@ class Context:
     def mbasis(self, F, order, shift):
         """Iterative approximant basis (nmod_poly_mat_mbasis)."""
         return self._basis(lib().pmlg_mbasis, F, order, shift)
+
+    def set_leaf_order(self, order):
+        """M-Basis / PM-Basis crossover (pmlg_pmbasis_set_leaf_order): 0 measures
+        it per shape for calls of order >= 512, 1..128 fixes it."""
+        _check(lib().pmlg_pmbasis_set_leaf_order(self.handle, int(order)))
+
+    def leaf_order(self, m, n, method="auto"):
+        """Leaf order a PM-Basis call of order >= 512 on an m x n input uses."""
+        t = ctypes.c_int64(0)
+        _check(lib().pmlg_pmbasis_leaf_order(self.handle, m, n, _method(method), ctypes.byref(t)))
+        return t.value
 
     def kernel_basis(self, F, shift=None, algo="zls", method="auto", cap=None):
         """Shift-ordered weak Popov basis of the left kernel of F

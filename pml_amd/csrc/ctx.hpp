@@ -116,6 +116,10 @@ struct pmlg_ctx_s {
     std::map<uint64_t, int> tuned;
     // measured refinement of the learned choice, per shape class
     std::map<uint64_t, int> refined;
+    // PM-Basis leaf order (M-Basis / PM-Basis crossover): fixed when > 0,
+    // else measured once per (m, n, method) and kept here
+    int leaf_order = 0;
+    std::map<uint64_t, int> leaf_orders;
     pmlg::DevBuf tune_io, tune_ws;
     // moduli without the roots a product needs: products modulo three FFT
     // primes (contexts created on first use) recombined by CRT

@@ -25,6 +25,7 @@
 // buffer and writing the other.  All control (nullity cases, basis length)
 // stays on the device: a leaf is a fixed sequence of launches.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <type_traits>
 
@@ -979,12 +980,100 @@ pmlg_status pmb_rec(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF,
     return PMLG_OK;
 }
 
-size_t arena_bytes(int64_t m, int64_t n, int64_t order)
+size_t arena_bytes(int64_t m, int64_t n, int64_t order, int64_t ordleaf)
 {
     // generous bound: a few copies of every level's operands
     const double lvl = (double)m * m * (order + 2) * 4 + (double)m * n * (order + 1) * 2;
-    const double leafb = (double)m * m * (kPmbasisThres + 2) * 2 + (double)m * n * 34 * 2 + m * m + 64;
+    const double leafb =
+        (double)m * m * (ordleaf + 2) * 2 + (double)m * n * (ordleaf + 2) * 2 + m * m + 64;
     return (size_t)((2.5 * lvl + leafb + 4096.0) * 8.0);
+}
+
+// M-Basis / PM-Basis crossover.  The basis does not depend on it: the second
+// half of the recursion repeats, on the residual of the first half's basis,
+// the steps that M-Basis would take from that basis on (each step depends only
+// on the current residual coefficient and shifts, and the basis updates are
+// row operations that commute with the product by the first half's basis), so
+// the leaf order is a pure speed choice.  Threshold t against 2t differs only
+// at the nodes of order 2t, split into two leaves of order t plus a middle
+// product and a product, or solved as one leaf: both are timed on seeded
+// synthetic input of the caller's shape, first for t = 32 (the reference's
+// threshold), then for t = 16 if the split won there, else for t = 64.
+// Measured once per context and (m, n, method), for orders of at least
+// kTuneMinOrder (below it the reference's threshold is used as it is).
+constexpr int kMaxLeafOrder = 128;
+constexpr int64_t kTuneMinOrder = 512;
+
+int64_t tune_leaf_bytes(int64_t m, int64_t n)
+{
+    return (int64_t)arena_bytes(m, n, kMaxLeafOrder, kMaxLeafOrder) + (m * n * kMaxLeafOrder + m) * 8 + 4096;
+}
+
+pmlg_status tune_leaf_order(pmlg_ctx ctx, int m, int n, int method, hipStream_t s, int *out)
+{
+    const uint64_t key = ((uint64_t)m << 16) | ((uint64_t)n << 8) | (uint64_t)(method & 0xff);
+    auto it = ctx->leaf_orders.find(key);
+    if (it != ctx->leaf_orders.end()) {
+        *out = it->second;
+        return PMLG_OK;
+    }
+    HIP_TRY(ctx->arena.ensure((size_t)tune_leaf_bytes(m, n)), "hipMalloc(arena)");
+    Arena ar;
+    ar.base = static_cast<char *>(ctx->arena.ptr);
+    ar.size = ctx->arena.bytes;
+    const int64_t len = kMaxLeafOrder;
+    uint64_t *dF = ar.alloc((size_t)(m * n * len));
+    int64_t *dShift = reinterpret_cast<int64_t *>(ar.alloc((size_t)m));
+    if (!dF || !dShift)
+        return fail(PMLG_ERR_ALLOC, "pmbasis: device arena exhausted");
+    std::vector<uint64_t> F((size_t)(m * n * len));
+    uint64_t x = 0x9e3779b97f4a7c15ull ^ key;
+    for (uint64_t &v : F) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        v = x % ctx->p;
+    }
+    HIP_TRY(hipMemcpyAsync(dF, F.data(), F.size() * 8, hipMemcpyHostToDevice, s), "H2D(tuning input)");
+    const std::vector<int64_t> zero((size_t)m, 0);
+    const size_t mark = ar.top;
+    // best of three runs (the first also loads the transform tables)
+    auto run = [&](int64_t order, int thres, double *sec) -> pmlg_status {
+        *sec = 1e30;
+        for (int r = 0; r < 3; ++r) {
+            HIP_TRY(hipMemcpyAsync(dShift, zero.data(), (size_t)m * 8, hipMemcpyHostToDevice, s),
+                    "H2D(shift)");
+            HIP_TRY(hipStreamSynchronize(s), "sync");
+            const auto t0 = std::chrono::steady_clock::now();
+            Poly P;
+            pmlg_status st = pmb_rec(ctx, ar, dF, len, len, m, n, order, dShift, method, thres, &P, s);
+            if (st != PMLG_OK)
+                return st;
+            HIP_TRY(hipStreamSynchronize(s), "sync");
+            *sec = std::min(*sec, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+            ar.top = mark;
+        }
+        return PMLG_OK;
+    };
+    // true when two leaves of order t (and the products) beat one of order 2t
+    auto split_wins = [&](int t, bool *wins) -> pmlg_status {
+        double leaf_s = 0, split_s = 0;
+        pmlg_status st = run(2 * t, 2 * t, &leaf_s);
+        if (st == PMLG_OK)
+            st = run(2 * t, t, &split_s);
+        *wins = split_s < leaf_s;
+        return st;
+    };
+    bool w32 = false, w = false;
+    pmlg_status st = split_wins(32, &w32);
+    if (st == PMLG_OK)
+        st = split_wins(w32 ? 16 : 64, &w);
+    if (st != PMLG_OK)
+        return st;
+    const int best = w32 ? (w ? 16 : 32) : (w ? 64 : kMaxLeafOrder);
+    ctx->leaf_orders[key] = best;
+    *out = best;
+    return PMLG_OK;
 }
 
 }  // namespace
@@ -1000,13 +1089,19 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
         return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
     if (capP < order + 1)
         return fail(PMLG_ERR_ARG, "approximant basis: output capacity must be at least order + 1");
+    int thres = ctx->leaf_order > 0 ? ctx->leaf_order : kPmbasisThres;
+    if (!iterative && ctx->leaf_order == 0 && order >= kTuneMinOrder) {
+        pmlg_status st = tune_leaf_order(ctx, (int)m, (int)n, method, s, &thres);
+        if (st != PMLG_OK)
+            return st;
+    }
     // the M-Basis kernels index one leaf's items in 32 bits
-    const int64_t ordleaf = iterative ? order : std::min<int64_t>(order, kPmbasisThres);
+    const int64_t ordleaf = iterative ? order : std::min<int64_t>(order, thres);
     if (m * m * (ordleaf + 1) + m * n * ordleaf >= ((int64_t)1 << 31))
         return fail(PMLG_ERR_UNSUPPORTED, "M-Basis: m^2 (order + 1) + m n order must stay below 2^31 "
                                           "(use PM-Basis for this order)");
     // device arena
-    size_t need = arena_bytes(m, n, order) +
+    size_t need = arena_bytes(m, n, order, ordleaf) +
                   (size_t)(m * m * (ordleaf + 2) * 2 + m * n * (ordleaf + 1) * 2) * 8;
     HIP_TRY(ctx->arena.ensure(need), "hipMalloc(arena)");
     Arena ar;
@@ -1020,8 +1115,7 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
         st = leaf(ctx, ar, dF, strideF, std::min(lenF, order), (int)m, (int)n, (int)order, dShift, &P,
                   s);
     else
-        st = pmb_rec(ctx, ar, dF, strideF, lenF, (int)m, (int)n, order, dShift, method, kPmbasisThres,
-                     &P, s);
+        st = pmb_rec(ctx, ar, dF, strideF, lenF, (int)m, (int)n, order, dShift, method, thres, &P, s);
     if (st != PMLG_OK)
         return st;
     HIP_TRY(hipMemcpyAsync(shift, dShift, (size_t)m * 8, hipMemcpyDeviceToHost, s), "D2H(shift)");
@@ -1141,6 +1235,39 @@ pmlg_status pmlg_mbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift
                         int64_t capP)
 {
     return basis_host(ctx, P, lenP, shift, F, m, n, lenF, order, capP, PMLG_METHOD_AUTO, true);
+}
+
+pmlg_status pmlg_pmbasis_set_leaf_order(pmlg_ctx ctx, int64_t order)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (order < 0 || order > kMaxLeafOrder)
+        return fail(PMLG_ERR_ARG, "pmbasis leaf order must be 0 (measured) or in [1, 128]");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->leaf_order = (int)order;
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_pmbasis_leaf_order(pmlg_ctx ctx, int64_t m, int64_t n, int method, int64_t *order)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (m <= 0 || n <= 0 || m > kMaxDim || n > kMaxDim || !order)
+        return fail(PMLG_ERR_ARG, "pmbasis leaf order: dimensions must be in [1, 128]");
+    if (!ctx->is_prime)
+        return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->leaf_order > 0) {
+        *order = ctx->leaf_order;
+        return PMLG_OK;
+    }
+    DeviceGuard g(ctx->device);
+    int t = kPmbasisThres;
+    st = tune_leaf_order(ctx, (int)m, (int)n, method, ctx->stream, &t);
+    *order = t;
+    return st;
 }
 
 }  // extern "C"

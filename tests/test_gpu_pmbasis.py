@@ -198,3 +198,45 @@ def test_pmbasis_dimension_sweep(pml, m, n, p):
     check(pml, p, F, order, shift, fn="mbasis")
     P = check(pml, p, F, order, shift)
     assert O.is_approximant_basis(P, F, order, shift, p) == 0
+
+
+@pytest.mark.parametrize("p", [P30, P60, 674830283164631843])
+def test_pmbasis_leaf_order_does_not_change_basis(pml, p):
+    """The M-Basis / PM-Basis crossover (PMBASIS_THRES,
+    nmod_poly_mat_approximant.h:101) is a speed choice only: whatever the leaf
+    order, the basis and shifts equal the oracle's (which splits at 32, as the
+    reference does), on generic and degenerate inputs; the non-FFT prime runs
+    the products through the multi-prime path."""
+    ctx = ctx_for(pml, p)
+    rng = np.random.default_rng(p % 7919)
+    try:
+        for kind, (m, n, order) in [("uniform", (6, 3, 300)), ("rkdef", (5, 2, 257)),
+                                    ("sparse", (4, 3, 200)), ("cdeg", (7, 4, 130))]:
+            F = make(rng, kind, m, n, order, p)
+            shift = rng.integers(-2, 5, m).astype(np.int64)
+            P_ref, s_ref = O.pmbasis(F, order, shift, p)
+            for t in (1, 7, 16, 64, 128):
+                ctx.set_leaf_order(t)
+                P_dev, s_dev = ctx.pmbasis(F, order, shift)
+                assert np.array_equal(O.normalise(P_dev), O.normalise(P_ref)), (kind, t)
+                assert np.array_equal(s_dev, s_ref), (kind, t)
+    finally:
+        ctx.set_leaf_order(0)
+
+
+def test_pmbasis_measured_leaf_order(pml):
+    """Calls of order >= 512 use the leaf order measured for their shape; the
+    basis is still the oracle's."""
+    ctx = ctx_for(pml, P30)
+    ctx.set_leaf_order(0)
+    t = ctx.leaf_order(8, 4)
+    assert t in (16, 32, 64, 128)
+    assert ctx.leaf_order(8, 4) == t  # kept per shape
+    rng = np.random.default_rng(53)
+    F = make(rng, "uniform", 8, 4, 600, P30)
+    shift = rng.integers(0, 3, 8).astype(np.int64)
+    P = check(pml, P30, F, 600, shift)
+    assert O.is_approximant_basis(P, F, 600, shift, P30) == 0
+    for bad in (-1, 129):
+        with pytest.raises(pml.PmlgError):
+            ctx.set_leaf_order(bad)
