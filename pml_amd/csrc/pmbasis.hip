@@ -111,8 +111,9 @@ __device__ uint64_t inv_euclid(uint64_t a, uint64_t p)
 {
     uint64_t r0 = p, r1 = a;
     int64_t s0 = 0, s1 = 1;
+    double d0 = (double)r0, d1 = (double)r1;  // fl(r0), fl(r1), carried along
     while (r1 != 0) {
-        const double f = (double)r0 / (double)r1;
+        const double f = d0 / d1;
         uint64_t q = (uint64_t)f;
         int64_t rem = (int64_t)(r0 - q * r1);
         if (f >= 0x1p49) {
@@ -129,7 +130,9 @@ __device__ uint64_t inv_euclid(uint64_t a, uint64_t p)
         }
         const int64_t s2 = s0 - (int64_t)q * s1;
         r0 = r1;
+        d0 = d1;
         r1 = (uint64_t)rem;
+        d1 = (double)r1;
         s0 = s1;
         s1 = s2;
     }
@@ -572,8 +575,54 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     }
     __syncthreads();
     // pivots of the reduced echelon form: 1 / At[prow[k]][pivcol[k]]
-    for (int k = tid; k < rank; k += kStepThreads)
-        inv[k] = ar.inv(At[prow[k] * ms + pivcol[k]]);
+    __shared__ V s_tinv;
+    if constexpr (sizeof(V) == 8) {
+        // 64-bit words: one extended Euclid for all pivots (Montgomery's
+        // simultaneous inversion) instead of one per pivot run in lockstep.
+        // Wave 0 forms, for every pivot, the product of all the others
+        // (lane l holds pivots l and l + 64, rank <= 128; prefix and suffix
+        // products by scans across the wave); wave 1 multiplies all pivots
+        // together and inverts that product.  1 / pivot k = inv[k] s_tinv,
+        // taken where it is used.  Pivots are nonzero, so is their product.
+        if (wave < 2) {
+            const V x0 = lane < rank ? At[prow[lane] * ms + pivcol[lane]] : ar.one();
+            const V x1 = lane + 64 < rank ? At[prow[lane + 64] * ms + pivcol[lane + 64]] : ar.one();
+            const V v = ar.mul(x0, x1);
+            if (wave == 0) {
+                V pre = v, suf = v;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const V up = __shfl_up(pre, d);
+                    const V dn = __shfl_down(suf, d);
+                    if (lane >= d)
+                        pre = ar.mul(pre, up);
+                    if (lane + d < 64)
+                        suf = ar.mul(suf, dn);
+                }
+                V ex = __shfl_up(pre, 1), exs = __shfl_down(suf, 1);
+                if (lane == 0)
+                    ex = ar.one();
+                if (lane == 63)
+                    exs = ar.one();
+                const V others = ar.mul(ex, exs);  // every other lane's pair
+                if (lane < rank)
+                    inv[lane] = ar.mul(others, x1);
+                if (lane + 64 < rank)
+                    inv[lane + 64] = ar.mul(others, x0);
+            } else {
+                V t = v;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1)
+                    t = ar.mul(t, __shfl_xor(t, d));
+                t = ar.inv(t);  // the same value in every lane
+                if (lane == 0)
+                    s_tinv = t;
+            }
+        }
+    } else {
+        for (int k = tid; k < rank; k += kStepThreads)
+            inv[k] = ar.inv(At[prow[k] * ms + pivcol[k]]);
+    }
     // permutation list of left_nullspace_compact: pivot columns, then the
     // others; the last wave (the inverses run on the first ones) counts the
     // pivot columns before each column by ballots
@@ -601,7 +650,10 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     if (mode == 1) {
         for (int idx = tid; idx < nullity * rank; idx += kStepThreads) {
             const int t = idx / rank, r = idx - t * rank;
-            const uint64_t v = ar.from(ar.mul(At[prow[r] * ms + plist[rank + t]], inv[r]));
+            V w = ar.mul(At[prow[r] * ms + plist[rank + t]], inv[r]);
+            if constexpr (sizeof(V) == 8)
+                w = ar.mul(w, s_tinv);
+            const uint64_t v = ar.from(w);
             a.X[(size_t)t * m + r] = v ? p - v : 0;
         }
         for (int i = tid; i < m; i += kStepThreads) {
