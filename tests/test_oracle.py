@@ -308,6 +308,36 @@ def test_mbasis_equals_pmbasis_generic():
             assert np.array_equal(s1, s2)
 
 
+def test_mbasis_equals_pmbasis_degenerate():
+    """The divide and conquer repeats M-Basis's steps exactly (each step
+    depends only on the residual coefficient and the shifts, and the updates
+    are row operations, which commute with the product by the first half's
+    basis), so the basis does not depend on where the recursion stops: the
+    iterative basis equals the recursive one also where pivots are scarce
+    (primes 2 to 7, mostly zero or repeated rows, zero columns, wide shifts)."""
+    rng = np.random.default_rng(1000)
+    for it in range(60):
+        p = (2, 3, 5, 7, 101, 998244353)[it % 6]
+        m, n = int(rng.integers(1, 9)), int(rng.integers(1, 9))
+        order = int(rng.integers(33, 130))
+        lenF = int(rng.integers(1, order + 10))
+        F = rng.integers(0, p, size=(m, n, lenF), dtype=np.uint64)
+        kind = it % 5
+        if kind == 1:
+            F[:, :, rng.random(lenF) < 0.7] = 0
+        if kind == 2 and m > 1:
+            F[m - 1] = F[0]
+        if kind == 3:
+            F[:, 0] = 0
+        if kind == 4:
+            F[rng.random((m, n, lenF)) < 0.9] = 0
+        sh = rng.integers(-5, 40, size=m).astype(np.int64)
+        P1, s1 = O.mbasis(F, order, sh, p)
+        P2, s2 = O.pmbasis(F, order, sh, p)
+        assert np.array_equal(O.normalise(P1), O.normalise(P2)), it
+        assert np.array_equal(s1, s2), it
+
+
 def test_golden_fixture_small_entries():
     """The committed fixture agrees with the oracle on the entries that are
     cheap to recompute (the others are checked on the GPU at full size)."""
