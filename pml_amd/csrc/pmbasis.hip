@@ -64,6 +64,11 @@ struct LeafArgs {
     uint64_t *app[2];  // m x m x cap, entry-major
     uint64_t *R[2];    // m x n x ordL, entry-major
     uint64_t *Rn;      // m x n: the coming step's residual coefficient
+    // Row scales.  The steps use unnormalised kernel vectors (own coefficient
+    // D, the product of the step's pivots, instead of 1), so basis row i is
+    // scale[i] times the normalised one; the leaf divides them out at its
+    // end.  scale[m] holds the current step's D (plain residue) for the update.
+    uint64_t *scale;   // m + 1 words
     uint64_t *X;       // nullity x rank, row stride m
     int *piv;          // m
     int *rowinfo;      // m: t >= 0 nullity row t, -1-r rank row r
@@ -574,54 +579,40 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         __syncthreads();
     }
     __syncthreads();
-    // pivots of the reduced echelon form: 1 / At[prow[k]][pivcol[k]]
-    __shared__ V s_tinv;
-    if constexpr (sizeof(V) == 8) {
-        // 64-bit words: one extended Euclid for all pivots (Montgomery's
-        // simultaneous inversion) instead of one per pivot run in lockstep.
-        // Wave 0 forms, for every pivot, the product of all the others
-        // (lane l holds pivots l and l + 64, rank <= 128; prefix and suffix
-        // products by scans across the wave); wave 1 multiplies all pivots
-        // together and inverts that product.  1 / pivot k = inv[k] s_tinv,
-        // taken where it is used.  Pivots are nonzero, so is their product.
-        if (wave < 2) {
-            const V x0 = lane < rank ? At[prow[lane] * ms + pivcol[lane]] : ar.one();
-            const V x1 = lane + 64 < rank ? At[prow[lane + 64] * ms + pivcol[lane + 64]] : ar.one();
-            const V v = ar.mul(x0, x1);
-            if (wave == 0) {
-                V pre = v, suf = v;
+    // Cofactors of the pivots of the reduced echelon form: for pivot k, the
+    // product of all the others (Montgomery's simultaneous-inversion trick
+    // without the inversion): the kernel vectors are then D times the
+    // normalised ones, D the product of all pivots, and no inverse is needed
+    // in the step.  Wave 0, lane l holding pivots l and l + 64 (rank <= 128):
+    // prefix and suffix products by scans across the wave.  Pivots are
+    // nonzero, so is D.
+    __shared__ V s_D;
+    if (wave == 0) {
+        const V x0 = lane < rank ? At[prow[lane] * ms + pivcol[lane]] : ar.one();
+        const V x1 = lane + 64 < rank ? At[prow[lane + 64] * ms + pivcol[lane + 64]] : ar.one();
+        const V v = ar.mul(x0, x1);
+        V pre = v, suf = v;
 #pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const V up = __shfl_up(pre, d);
-                    const V dn = __shfl_down(suf, d);
-                    if (lane >= d)
-                        pre = ar.mul(pre, up);
-                    if (lane + d < 64)
-                        suf = ar.mul(suf, dn);
-                }
-                V ex = __shfl_up(pre, 1), exs = __shfl_down(suf, 1);
-                if (lane == 0)
-                    ex = ar.one();
-                if (lane == 63)
-                    exs = ar.one();
-                const V others = ar.mul(ex, exs);  // every other lane's pair
-                if (lane < rank)
-                    inv[lane] = ar.mul(others, x1);
-                if (lane + 64 < rank)
-                    inv[lane + 64] = ar.mul(others, x0);
-            } else {
-                V t = v;
-#pragma unroll
-                for (int d = 32; d >= 1; d >>= 1)
-                    t = ar.mul(t, __shfl_xor(t, d));
-                t = ar.inv(t);  // the same value in every lane
-                if (lane == 0)
-                    s_tinv = t;
-            }
+        for (int d = 1; d < 64; d <<= 1) {
+            const V up = __shfl_up(pre, d);
+            const V dn = __shfl_down(suf, d);
+            if (lane >= d)
+                pre = ar.mul(pre, up);
+            if (lane + d < 64)
+                suf = ar.mul(suf, dn);
         }
-    } else {
-        for (int k = tid; k < rank; k += kStepThreads)
-            inv[k] = ar.inv(At[prow[k] * ms + pivcol[k]]);
+        V ex = __shfl_up(pre, 1), exs = __shfl_down(suf, 1);
+        if (lane == 0)
+            ex = ar.one();
+        if (lane == 63)
+            exs = ar.one();
+        const V others = ar.mul(ex, exs);  // every other lane's pair
+        if (lane < rank)
+            inv[lane] = ar.mul(others, x1);
+        if (lane + 64 < rank)
+            inv[lane + 64] = ar.mul(others, x0);
+        if (lane == 63)
+            s_D = pre;
     }
     // permutation list of left_nullspace_compact: pivot columns, then the
     // others; the last wave (the inverses run on the first ones) counts the
@@ -650,10 +641,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     if (mode == 1) {
         for (int idx = tid; idx < nullity * rank; idx += kStepThreads) {
             const int t = idx / rank, r = idx - t * rank;
-            V w = ar.mul(At[prow[r] * ms + plist[rank + t]], inv[r]);
-            if constexpr (sizeof(V) == 8)
-                w = ar.mul(w, s_tinv);
-            const uint64_t v = ar.from(w);
+            const uint64_t v = ar.from(ar.mul(At[prow[r] * ms + plist[rank + t]], inv[r]));
             a.X[(size_t)t * m + r] = v ? p - v : 0;
         }
         for (int i = tid; i < m; i += kStepThreads) {
@@ -662,11 +650,22 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             a.rowinfo[row] = i < rank ? -1 - i : i - rank;
             if (i < rank)
                 a.shift[row] = sh[row] + 1;
+            // rows updated by a kernel vector take the factor D; the scales
+            // start at 1 (Montgomery form) at the leaf's first step
+            if (ord == 0 || i >= rank) {
+                const V s0 = ord == 0 ? ar.one() : (V)a.scale[row];
+                a.scale[row] = i >= rank ? ar.mul(s0, s_D) : s0;
+            }
         }
+        if (tid == 0)
+            a.scale[m] = ar.from(s_D);
     } else if (mode == 2) {
         for (int i = tid; i < m; i += kStepThreads)
             a.shift[i] = sh[i] + (a.ordL - ord);
     }
+    if (mode != 1 && ord == 0)
+        for (int i = tid; i < m; i += kStepThreads)
+            a.scale[i] = ar.one();
     if (tid == 0) {
         const int len = c[C_LEN] + c[C_GROW];
         c[C_CUR] = cur;
@@ -782,6 +781,14 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
     __syncthreads();
     const int rank = a.ctl[C_RANK];
     const int L = a.ctl[C_LEN];
+    // the kernel vectors' own coefficient D (the step's pivot product)
+    const uint64_t D = a.scale[m];
+    auto own = [&](uint64_t x) -> uint64_t {
+        if constexpr (SMALL)
+            return barrett(x * D, a.mc.bm, a.mc.p);  // x, D < 2^30
+        else
+            return mulmod_g(x, D, a.mc);
+    };
     const unsigned L1 = (unsigned)(L + 1 <= cap ? L + 1 : cap);
     const unsigned items = (unsigned)(m * m) * L1;
     // residual series: coefficients ord+1 .. ordL-1
@@ -800,7 +807,7 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
             const int ri = rinfo[i];
             uint64_t v;
             if (ri >= 0) {
-                v = d < L ? row_combo<SMALL>(src, a, piv, ri, rank, cap, m, j, d, src[(size_t)e * cap + d])
+                v = d < L ? row_combo<SMALL>(src, a, piv, ri, rank, cap, m, j, d, own(src[(size_t)e * cap + d]))
                           : 0;
             } else {
                 v = d >= 1 ? src[(size_t)e * cap + d - 1] : 0;
@@ -815,12 +822,38 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
             const int i = (int)(e / (unsigned)n), j = (int)(e - (unsigned)i * n);
             const int ri = rinfo[i];
             const uint64_t v =
-                ri >= 0 ? row_combo<SMALL>(rs, a, piv, ri, rank, a.ordL, n, j, d, rs[(size_t)e * a.ordL + d])
+                ri >= 0 ? row_combo<SMALL>(rs, a, piv, ri, rank, a.ordL, n, j, d, own(rs[(size_t)e * a.ordL + d]))
                         : rs[(size_t)e * a.ordL + d - 1];
             rd[(size_t)e * a.ordL + d] = v;
             if (d == ord + 1)
                 a.Rn[e] = v;  // the coming step's coefficient
         }
+    }
+}
+
+// scale[i] <- 1 / scale[i] as a plain residue (the scales are Montgomery
+// residues, nonzero as products of pivots)
+template <typename Arith>
+__global__ void __launch_bounds__(kMaxDim) leaf_scale_inv_kernel(LeafArgs a)
+{
+    const Arith ar(a.mc);
+    const int i = threadIdx.x;
+    if (i < a.m)
+        a.scale[i] = ar.from(ar.inv((typename Arith::V)a.scale[i]));
+}
+
+// coefficients [0, len) of the m x m basis (entries of capacity cap) times
+// their row's inverse scale
+__global__ void __launch_bounds__(256) leaf_rescale_kernel(uint64_t *__restrict__ P, int64_t cap, int m,
+                                                           int64_t len, const uint64_t *__restrict__ iscale,
+                                                           ModConst mc)
+{
+    const int64_t items = (int64_t)m * m * len;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < items;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = idx / len, d = idx - e * len;
+        const int row = (int)(e / m);
+        P[e * cap + d] = mulmod_g(P[e * cap + d], iscale[row], mc);
     }
 }
 
@@ -914,9 +947,10 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     a.R[0] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
     a.R[1] = ar.alloc((size_t)m * n * (ordL > 0 ? ordL : 1));
     a.Rn = ar.alloc((size_t)m * n);
+    a.scale = ar.alloc((size_t)m + 1);
     a.X = ar.alloc((size_t)m * m + kXPad);
     uint64_t *ints = ar.alloc((size_t)m + C_WORDS + 8);
-    if (!a.app[0] || !a.app[1] || !a.R[0] || !a.R[1] || !a.Rn || !a.X || !ints)
+    if (!a.app[0] || !a.app[1] || !a.R[0] || !a.R[1] || !a.Rn || !a.scale || !a.X || !ints)
         return fail(PMLG_ERR_ALLOC, "pmbasis: device arena exhausted");
     a.piv = reinterpret_cast<int *>(ints);
     a.rowinfo = a.piv + m;
@@ -972,6 +1006,16 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     HIP_TRY(hipStreamSynchronize(s), "leaf sync");
     const int res = ctl[C_CUR] ^ ctl[C_FLIP];
     const int L = ctl[C_LEN] + ctl[C_GROW];
+    if (ordL > 0) {
+        // divide the row scales out of the basis
+        if (small)
+            hipLaunchKernelGGL(leaf_scale_inv_kernel<Mont32Arith>, dim3(1), dim3(kMaxDim), 0, s, a);
+        else
+            hipLaunchKernelGGL(leaf_scale_inv_kernel<Mont64Arith>, dim3(1), dim3(kMaxDim), 0, s, a);
+        hipLaunchKernelGGL(leaf_rescale_kernel, dim3(grid_for((int64_t)m * m * L)), dim3(256), 0, s,
+                           a.app[res], (int64_t)cap, m, (int64_t)L, a.scale, ctx->mc);
+        HIP_TRY(hipGetLastError(), "leaf_rescale_kernel");
+    }
     // result to the mark (app[0] sits there; move app[1] down if needed)
     out->d = a.app[0];
     out->cap = cap;
