@@ -23,7 +23,13 @@
 // Kernel 1 (one workgroup) does 2-4 and writes the step's description;
 // kernel 2 (a grid) applies the row operations to appbas and R, reading one
 // buffer and writing the other.  All control (nullity cases, basis length)
-// stays on the device: a leaf is a fixed sequence of launches.
+// stays on the device: a leaf is a fixed sequence of launches.  The steps
+// skip the division by the pivots in 3: with D the product of the pivots,
+// they apply D X and multiply the updated rows by D, so each basis row is the
+// reference's row times a known nonzero scale, and the leaf divides the
+// scales out at its end.  The step's decisions only read which entries are
+// zero, which row scaling keeps, and the normalised kernel of the scaled
+// residual, applied to the scaled rows, gives the scaled normalised update.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -329,8 +335,8 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // 64-bit words first (sh), then the V arrays, then the int arrays
     int64_t *sh = reinterpret_cast<int64_t *>(smem);             // kMaxDim
     V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // n x ms
-    V *inv = At + n * ms;                                        // kMaxDim
-    V *Ck = inv + kMaxDim;                                            // 2 x kMaxDim x (P + 1)
+    V *cof = At + n * ms;                                        // kMaxDim: pivot cofactors
+    V *Ck = cof + kMaxDim;                                            // 2 x kMaxDim x (P + 1)
     int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxDim * (P + 1));  // kMaxDim
     int *pivcol = perm + kMaxDim;                                // kMaxDim
     int *plist = pivcol + kMaxDim;                               // kMaxDim
@@ -608,14 +614,14 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             exs = ar.one();
         const V others = ar.mul(ex, exs);  // every other lane's pair
         if (lane < rank)
-            inv[lane] = ar.mul(others, x1);
+            cof[lane] = ar.mul(others, x1);
         if (lane + 64 < rank)
-            inv[lane + 64] = ar.mul(others, x0);
+            cof[lane + 64] = ar.mul(others, x0);
         if (lane == 63)
             s_D = pre;
     }
     // permutation list of left_nullspace_compact: pivot columns, then the
-    // others; the last wave (the inverses run on the first ones) counts the
+    // others; the last wave (the cofactors run on the first one) counts the
     // pivot columns before each column by ballots
     if (tid >= kStepThreads - 64) {
         const int l = tid - (kStepThreads - 64);
@@ -641,7 +647,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     if (mode == 1) {
         for (int idx = tid; idx < nullity * rank; idx += kStepThreads) {
             const int t = idx / rank, r = idx - t * rank;
-            const uint64_t v = ar.from(ar.mul(At[prow[r] * ms + plist[rank + t]], inv[r]));
+            const uint64_t v = ar.from(ar.mul(At[prow[r] * ms + plist[rank + t]], cof[r]));
             a.X[(size_t)t * m + r] = v ? p - v : 0;
         }
         for (int i = tid; i < m; i += kStepThreads) {
@@ -962,7 +968,7 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
                        dim3(256), 0, s, a, F, strideF, lt);
     HIP_TRY(hipGetLastError(), "leaf_init_kernel");
     // LDS of the step kernel: shifts, At (n rows of odd stride m | 1),
-    // inverses, two coefficient buffers, then 7 int arrays (sizes as in the kernel)
+    // pivot cofactors, two coefficient buffers, then 7 int arrays (sizes as in the kernel)
     const size_t lds = ((size_t)n * (m | 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 + 7 * kMaxDim * sizeof(int);
     static bool attr = false;
     if (!attr) {
