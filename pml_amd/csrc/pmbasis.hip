@@ -73,7 +73,8 @@ struct LeafArgs {
     // Row scales.  The steps use unnormalised kernel vectors (own coefficient
     // D, the product of the step's pivots, instead of 1), so basis row i is
     // scale[i] times the normalised one; the leaf divides them out at its
-    // end.  scale[m] holds the current step's D (plain residue) for the update.
+    // end.  scale[m] holds the current step's D for the update (plain residue
+    // for 64-bit words, Montgomery residue D 2^32 mod p for 32-bit ones).
     uint64_t *scale;   // m + 1 words
     uint64_t *X;       // nullity x rank, row stride m
     int *piv;          // m
@@ -664,7 +665,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             }
         }
         if (tid == 0)
-            a.scale[m] = ar.from(s_D);
+            a.scale[m] = sizeof(V) == 8 ? ar.from(s_D) : (uint64_t)s_D;
     } else if (mode == 2) {
         for (int i = tid; i < m; i += kStepThreads)
             a.shift[i] = sh[i] + (a.ordL - ord);
@@ -790,8 +791,8 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
     // the kernel vectors' own coefficient D (the step's pivot product)
     const uint64_t D = a.scale[m];
     auto own = [&](uint64_t x) -> uint64_t {
-        if constexpr (SMALL)
-            return barrett(x * D, a.mc.bm, a.mc.p);  // x, D < 2^30
+        if constexpr (SMALL)  // D in Montgomery form: REDC(x D 2^32) = x D mod p
+            return Mont32Arith(a.mc).redc((uint64_t)(uint32_t)x * (uint32_t)D);
         else
             return mulmod_g(x, D, a.mc);
     };

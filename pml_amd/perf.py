@@ -64,6 +64,7 @@ def run_pmbasis(a):
     m, n, order = 64, 32, 2048
     p = a.p
     ctx = Context(p)
+    ctx.set_leaf_order(a.leaf_order)
     rng = np.random.default_rng(401)
     F = rng.integers(0, p, size=(m, n, order), dtype=np.uint64)
     shift = np.zeros(m, dtype=np.int64)
@@ -93,7 +94,8 @@ def run_pmbasis(a):
     out = {"workload": "pmbasis 64x32 order 2048 p=%d shift 0" % p, "seconds_per_basis": round(dt, 5),
            "bases_per_s": round(1.0 / dt, 3), "seconds_per_basis_device_resident": round(dt_dev, 5),
            "device_resident_identical": bool(same), "basis_length": int(P.shape[2]),
-           "shift_out_sum": int(sh.sum()), "residual_zero": bool(not R.any()), "reps": a.reps}
+           "shift_out_sum": int(sh.sum()), "residual_zero": bool(not R.any()), "reps": a.reps,
+           "leaf_order": ctx.leaf_order(m, n)}
     print(json.dumps(out))
     return 0
 
@@ -206,6 +208,8 @@ def main(argv=None):
     p = sub.add_parser("pmbasis")
     p.add_argument("--reps", type=int, default=3)
     p.add_argument("--p", type=int, default=P30, help="modulus (default 998244353)")
+    p.add_argument("--leaf-order", type=int, default=0,
+                   help="M-Basis/PM-Basis crossover: 0 measures it (default), 1..128 fixes it")
     mu = sub.add_parser("mul")
     mu.add_argument("--reps", type=int, default=20)
     mu.add_argument("--p", type=int, default=P60, help="modulus (default 2^60-3*2^24+1)")
