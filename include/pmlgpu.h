@@ -124,7 +124,7 @@ pmlg_status pmlg_polmat_mulmid_entries(pmlg_ctx ctx, uint64_t *const *out, int64
  * P on return.  1 <= m, n <= 128.  The products inside PM-Basis run on the
  * device with the given pointwise method.  pmlg_mbasis returns
  * PMLG_ERR_UNSUPPORTED when m^2 (order + 1) + m n order >= 2^31 (PM-Basis
- * has no such limit: its M-Basis leaves have order at most 128). */
+ * has no such limit: its M-Basis leaves have order at most 256). */
 pmlg_status pmlg_pmbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
                          const uint64_t *F, int64_t m, int64_t n, int64_t lenF, int64_t order,
                          int64_t capP, int method);
@@ -143,9 +143,9 @@ pmlg_status pmlg_pmbasis_dev(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *
  * a compile-time 32 in the reference).  The basis is the same for every leaf
  * order; only the time changes.  order 0 (the default): calls of order >= 512
  * use a leaf order measured once per context and (m, n, method) among 16, 32,
- * 64 and 128 (two leaves of order t plus their products against one leaf of
- * order 2t, timed on seeded synthetic input), smaller calls use 32;
- * order in [1, 128]: that leaf order for every call. */
+ * 64, 128 and 256 (two leaves of order t plus their products against one leaf
+ * of order 2t, timed on seeded synthetic input), smaller calls use 32;
+ * order in [1, 256]: that leaf order for every call. */
 pmlg_status pmlg_pmbasis_set_leaf_order(pmlg_ctx ctx, int64_t order);
 /* The leaf order a PM-Basis call of order >= 512 on an m x n input would use
  * (measured now if it is not yet known). */

@@ -325,7 +325,7 @@ class Context:
 
     def set_leaf_order(self, order):
         """M-Basis / PM-Basis crossover (pmlg_pmbasis_set_leaf_order): 0 measures
-        it per shape for calls of order >= 512, 1..128 fixes it."""
+        it per shape for calls of order >= 512, 1..256 fixes it."""
         _check(lib().pmlg_pmbasis_set_leaf_order(self.handle, int(order)))
 
     def leaf_order(self, m, n, method="auto"):

@@ -1101,10 +1101,11 @@ size_t arena_bytes(int64_t m, int64_t n, int64_t order, int64_t ordleaf)
 // at the nodes of order 2t, split into two leaves of order t plus a middle
 // product and a product, or solved as one leaf: both are timed on seeded
 // synthetic input of the caller's shape, first for t = 32 (the reference's
-// threshold), then for t = 16 if the split won there, else for t = 64.
-// Measured once per context and (m, n, method), for orders of at least
-// kTuneMinOrder (below it the reference's threshold is used as it is).
-constexpr int kMaxLeafOrder = 128;
+// threshold), then for t = 16 if the split won there, else for t = 64, and
+// then for t = 128 if the leaf of 128 won too.  Measured once per context and
+// (m, n, method), for orders of at least kTuneMinOrder (below it the
+// reference's threshold is used as it is).
+constexpr int kMaxLeafOrder = 256;
 constexpr int64_t kTuneMinOrder = 512;
 
 int64_t tune_leaf_bytes(int64_t m, int64_t n)
@@ -1173,7 +1174,13 @@ pmlg_status tune_leaf_order(pmlg_ctx ctx, int m, int n, int method, hipStream_t 
         st = split_wins(w32 ? 16 : 64, &w);
     if (st != PMLG_OK)
         return st;
-    const int best = w32 ? (w ? 16 : 32) : (w ? 64 : kMaxLeafOrder);
+    int best = w32 ? (w ? 16 : 32) : (w ? 64 : 0);
+    if (best == 0) {
+        st = split_wins(128, &w);
+        if (st != PMLG_OK)
+            return st;
+        best = w ? 128 : 256;
+    }
     ctx->leaf_orders[key] = best;
     *out = best;
     return PMLG_OK;
@@ -1346,7 +1353,7 @@ pmlg_status pmlg_pmbasis_set_leaf_order(pmlg_ctx ctx, int64_t order)
     if (st != PMLG_OK)
         return st;
     if (order < 0 || order > kMaxLeafOrder)
-        return fail(PMLG_ERR_ARG, "pmbasis leaf order must be 0 (measured) or in [1, 128]");
+        return fail(PMLG_ERR_ARG, "pmbasis leaf order must be 0 (measured) or in [1, 256]");
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->leaf_order = (int)order;
     return PMLG_OK;

@@ -209,7 +209,7 @@ def main(argv=None):
     p.add_argument("--reps", type=int, default=3)
     p.add_argument("--p", type=int, default=P30, help="modulus (default 998244353)")
     p.add_argument("--leaf-order", type=int, default=0,
-                   help="M-Basis/PM-Basis crossover: 0 measures it (default), 1..128 fixes it")
+                   help="M-Basis/PM-Basis crossover: 0 measures it (default), 1..256 fixes it")
     mu = sub.add_parser("mul")
     mu.add_argument("--reps", type=int, default=20)
     mu.add_argument("--p", type=int, default=P60, help="modulus (default 2^60-3*2^24+1)")

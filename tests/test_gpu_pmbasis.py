@@ -215,7 +215,7 @@ def test_pmbasis_leaf_order_does_not_change_basis(pml, p):
             F = make(rng, kind, m, n, order, p)
             shift = rng.integers(-2, 5, m).astype(np.int64)
             P_ref, s_ref = O.pmbasis(F, order, shift, p)
-            for t in (1, 7, 16, 64, 128):
+            for t in (1, 7, 16, 64, 128, 256):
                 ctx.set_leaf_order(t)
                 P_dev, s_dev = ctx.pmbasis(F, order, shift)
                 assert np.array_equal(O.normalise(P_dev), O.normalise(P_ref)), (kind, t)
@@ -230,13 +230,13 @@ def test_pmbasis_measured_leaf_order(pml):
     ctx = ctx_for(pml, P30)
     ctx.set_leaf_order(0)
     t = ctx.leaf_order(8, 4)
-    assert t in (16, 32, 64, 128)
+    assert t in (16, 32, 64, 128, 256)
     assert ctx.leaf_order(8, 4) == t  # kept per shape
     rng = np.random.default_rng(53)
     F = make(rng, "uniform", 8, 4, 600, P30)
     shift = rng.integers(0, 3, 8).astype(np.int64)
     P = check(pml, P30, F, 600, shift)
     assert O.is_approximant_basis(P, F, 600, shift, P30) == 0
-    for bad in (-1, 129):
+    for bad in (-1, 257):
         with pytest.raises(pml.PmlgError):
             ctx.set_leaf_order(bad)
