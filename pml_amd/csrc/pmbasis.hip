@@ -312,7 +312,9 @@ __device__ __forceinline__ uint64_t lane_value(uint64_t v, int l)
 // stay where they are (prow[k] is the row of the k-th pivot, pivcol[k] its
 // column); for a pivot at (pr, c) every other row r becomes
 // pv * row_r - At[r][c] * row_pr with pv = At[pr][c].  The k-th row of the
-// reduced row echelon form is row prow[k] divided by its pivot.
+// reduced row echelon form is row prow[k] divided by its pivot; the step
+// multiplies by the other pivots' product (the pivot's cofactor) instead, so
+// its kernel vectors are the normalised ones times the product of all pivots.
 //
 // The columns are taken P at a time (a panel).  One wave runs the pivot
 // steps of the panel on the panel's columns only and composes them: every row
