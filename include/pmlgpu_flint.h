@@ -27,10 +27,11 @@
  *
  * Types: with PMLG_USE_FLINT defined, FLINT's own nmod_poly_mat.h is used
  * (source compatibility; the struct layout is FLINT's).  Otherwise a subset
- * with FLINT's field names is declared here, with the FLINT <= 3.2 layout of
- * nmod_poly_mat_struct (entries, r, c, rows, modulus), and the few FLINT
- * memory-management calls the drop-in needs are provided as pmlg_-prefixed
- * functions mapped to FLINT's names by macros.
+ * with FLINT's field names is declared here, with the layout of FLINT >= 3.3
+ * (nmod_poly_mat_struct = entries, r, c, modulus; nmod_poly_struct = coeffs,
+ * alloc, length, mod), so that matrices built by a FLINT 3.4 caller are read
+ * as they are; the few FLINT memory-management calls the drop-in needs are
+ * provided as pmlg_-prefixed functions mapped to FLINT's names by macros.
  */
 #ifndef PMLGPU_FLINT_H
 #define PMLGPU_FLINT_H
@@ -63,16 +64,20 @@ typedef struct {
 } nmod_poly_struct;
 typedef nmod_poly_struct nmod_poly_t[1];
 
+/* FLINT >= 3.3 layout (the reference needs FLINT >= 3.4, README.md:9):
+ * entries row-major and contiguous, no `rows` array; entry (i, j) is
+ * entries + i*c + j (nmod_poly_mat_utils.c:34-49 and
+ * nmod_mat_poly_mbasis.c:321-345 take this branch from FLINT 3.3 on).
+ * 32 bytes: entries at 0, r at 8, c at 16, modulus at 24. */
 typedef struct {
     nmod_poly_struct *entries;
     slong r;
     slong c;
-    nmod_poly_struct **rows;
     ulong modulus;
 } nmod_poly_mat_struct;
 typedef nmod_poly_mat_struct nmod_poly_mat_t[1];
 
-#define nmod_poly_mat_entry(mat, i, j) ((mat)->rows[i] + (j))
+#define nmod_poly_mat_entry(mat, i, j) ((mat)->entries + (i) * (mat)->c + (j))
 #define nmod_poly_mat_nrows(mat) ((mat)->r)
 #define nmod_poly_mat_ncols(mat) ((mat)->c)
 
@@ -139,12 +144,22 @@ void nmod_poly_mat_pmbasis(nmod_poly_mat_t appbas, slong *shift, const nmod_poly
 void nmod_poly_mat_mbasis(nmod_poly_mat_t appbas, slong *shift, const nmod_poly_mat_t F,
                           ulong order);
 /* Certificate (ROW_LOWER only, as the reference): square; shift-ordered weak
- * Popov; appbas*F = 0 mod x^order (residual on the device); det(appbas) of
- * degree sum of diagonal degrees (checked as det = lambda x^D at x = 1 and
- * three seeded points, determinants on the device); [appbas(0) | coefficient
- * `order` of appbas*F] of full rank (rank on the device). */
+ * Popov; appbas*F = 0 mod x^order (residual on the device); deg det(appbas)
+ * equal to the sum D of the diagonal degrees, as the reference tests it
+ * (verification.c:73-84: the degree only, not that det is a monomial); and
+ * [appbas(0) | coefficient `order` of appbas*F] of full rank (rank on the
+ * device).  For a square shift-ordered weak Popov matrix deg det = D holds by
+ * construction (its shifted leading matrix is triangular with a nonzero
+ * diagonal); otherwise the degree is found exactly by device determinants at
+ * sum-of-row-degrees + 1 points and interpolation, when p exceeds that count
+ * (else its message is not printed).  The result is the reference's. */
 int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_poly_mat_t F,
                                        slong order, const slong *shift, orientation_t orient);
+
+/* Number of device contexts the drop-in keeps (one per modulus, at most 32,
+ * least recently used replaced first; no reference counterpart, a
+ * diagnostic for tests). */
+int pmlg_dropin_cached_contexts(void);
 
 /* Minimal kernel bases (nmod_poly_mat_kernel.h:81-105), computed on the
  * device (pmlg_kernel_basis): the basis occupies the first `nullity` rows

@@ -888,28 +888,61 @@ int orc_is_approximant_basis(const uint64_t *P, int64_t lenP,
                 break;
             }
 
-    /* det(P) = lambda * x^D with D the sum of diagonal degrees: checked at
-     * x = 1 (gives lambda) and three seeded random points */
+    /* deg det(P) == D with D the sum of the diagonal degrees, the degree
+     * test of verification.c:73-84 (not a test that det is a monomial).
+     * det(P) has degree at most Dmax, the sum of the row degrees, so its
+     * values at 0..Dmax fix it: the degree is the index of the last nonzero
+     * Newton divided difference.  A shifted ordered weak Popov P has
+     * deg det = D by construction (its shifted leading matrix is lower
+     * triangular with a nonzero diagonal), so the values are computed only
+     * for a P that failed that test, when p > Dmax (distinct points) and the
+     * work is small; otherwise the degree is left undecided (bit 4 not set). */
     if (m > 0)
     {
-        int64_t D = 0;
+        int64_t D = 0, dmax = 0, zero_row = 0;
         for (int64_t i = 0; i < m; i++)
-            D += poly_degree(P + (i * m + i) * lenP, lenP);
-        uint64_t *E = malloc((size_t)(m * m) * sizeof(uint64_t));
-        orc_polmat_eval(E, P, m, m, lenP, 1 % p, p);
-        uint64_t lambda = det_const(E, m, p);
-        if (lambda == 0 || D < 0)
-            fail |= 4;
-        uint64_t st = seed;
-        for (int t = 0; t < 3 && !(fail & 4); t++)
         {
-            uint64_t x = p > 2 ? 2 + orc_splitmix64(&st) % (p - 2) : 1 % p;
-            orc_polmat_eval(E, P, m, m, lenP, x, p);
-            uint64_t dv = det_const(E, m, p);
-            if (dv != orc_mulmod(lambda, orc_powmod(x, (uint64_t)D, p), p))
-                fail |= 4;
+            D += poly_degree(P + (i * m + i) * lenP, lenP);
+            int64_t r = -1;
+            for (int64_t j = 0; j < m; j++)
+            {
+                const int64_t d = poly_degree(P + (i * m + j) * lenP, lenP);
+                r = d > r ? d : r;
+            }
+            if (r < 0)
+                zero_row = 1;
+            dmax += r > 0 ? r : 0;
         }
-        free(E);
+        int64_t deg = -2;
+        if (!(fail & 1))
+            deg = D;
+        else if (zero_row)
+            deg = -1;
+        else if ((uint64_t)dmax < p &&
+                 (double)(dmax + 1) * (double)(m * m) * (double)(lenP + m) < 0x1p30)
+        {
+            uint64_t *E = malloc((size_t)(m * m) * sizeof(uint64_t));
+            uint64_t *a = malloc((size_t)(dmax + 1) * sizeof(uint64_t));
+            for (int64_t x = 0; x <= dmax; x++)
+            {
+                orc_polmat_eval(E, P, m, m, lenP, (uint64_t)x, p);
+                a[x] = det_const(E, m, p);
+            }
+            for (int64_t j = 1; j <= dmax; j++)
+            {
+                const uint64_t ij = orc_invmod((uint64_t)j, p);
+                for (int64_t i = dmax; i >= j; i--)
+                    a[i] = orc_mulmod(submod(a[i], a[i - 1], p), ij, p);
+            }
+            deg = -1;
+            for (int64_t j = 0; j <= dmax; j++)
+                if (a[j])
+                    deg = j;
+            free(a);
+            free(E);
+        }
+        if (deg != -2 && deg != D)
+            fail |= 4;
     }
 
     /* [P(0) | C] full rank, C = coefficient `order` of P*F */

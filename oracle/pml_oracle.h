@@ -102,7 +102,10 @@ void orc_pmbasis(uint64_t **P, int64_t *lenP, int64_t *shift,
 /* Certificate of verification.c:21-116 (orientation ROW_LOWER).  Returns a
  * bit mask of failed checks (0 = approximant basis):
  *   1 = not ordered weak Popov, 2 = residual not 0 mod x^order,
- *   4 = det not lambda*x^D (randomised, 3 points), 8 = [P(0) | C] not full rank. */
+ *   4 = deg det(P) differs from the sum D of the diagonal degrees (exact, as
+ *       verification.c:73-84 tests it; undecided, so not set, for a P that is
+ *       not weak Popov when p <= sum of row degrees or the work is large),
+ *   8 = [P(0) | C] not full rank.  `seed` is unused (kept for callers). */
 int orc_is_approximant_basis(const uint64_t *P, int64_t lenP,
                              const uint64_t *F, int64_t m, int64_t n, int64_t lenF,
                              int64_t order, const int64_t *shift, uint64_t p,

@@ -26,6 +26,22 @@
 
 #include "../../include/pmlgpu_flint.h"
 
+#ifndef PMLG_USE_FLINT
+#include <stddef.h>
+/* the FLINT >= 3.3 layouts a FLINT 3.4 caller hands over (x86-64 LP64) */
+_Static_assert(sizeof(nmod_t) == 24, "nmod_t is three words");
+_Static_assert(sizeof(nmod_poly_struct) == 48 && offsetof(nmod_poly_struct, coeffs) == 0 &&
+                   offsetof(nmod_poly_struct, alloc) == 8 &&
+                   offsetof(nmod_poly_struct, length) == 16 &&
+                   offsetof(nmod_poly_struct, mod) == 24,
+               "nmod_poly_struct layout");
+_Static_assert(sizeof(nmod_poly_mat_struct) == 32 && offsetof(nmod_poly_mat_struct, entries) == 0 &&
+                   offsetof(nmod_poly_mat_struct, r) == 8 &&
+                   offsetof(nmod_poly_mat_struct, c) == 16 &&
+                   offsetof(nmod_poly_mat_struct, modulus) == 24,
+               "nmod_poly_mat_struct layout (FLINT >= 3.3: no rows array)");
+#endif
+
 /* ------------------------------------------------------------------------ */
 /* memory management of the FLINT subset                                     */
 /* ------------------------------------------------------------------------ */
@@ -103,7 +119,6 @@ ulong pmlg_nmod_poly_get_coeff_ui(const nmod_poly_t poly, slong j)
 void pmlg_nmod_poly_mat_init(nmod_poly_mat_t mat, slong rows, slong cols, ulong n)
 {
     mat->entries = NULL;
-    mat->rows = NULL;
     mat->r = rows;
     mat->c = cols;
     mat->modulus = n;
@@ -114,13 +129,6 @@ void pmlg_nmod_poly_mat_init(nmod_poly_mat_t mat, slong rows, slong cols, ulong 
         for (slong e = 0; e < rows * cols; e++)
             pmlg_nmod_poly_init(mat->entries + e, n);
     }
-    if (rows > 0) {
-        mat->rows = (nmod_poly_struct **)malloc((size_t)rows * sizeof(nmod_poly_struct *));
-        if (!mat->rows)
-            abort();
-        for (slong i = 0; i < rows; i++)
-            mat->rows[i] = mat->entries ? mat->entries + i * cols : NULL;
-    }
 }
 
 void pmlg_nmod_poly_mat_clear(nmod_poly_mat_t mat)
@@ -129,9 +137,7 @@ void pmlg_nmod_poly_mat_clear(nmod_poly_mat_t mat)
         for (slong e = 0; e < mat->r * mat->c; e++)
             pmlg_nmod_poly_clear(mat->entries + e);
     free(mat->entries);
-    free(mat->rows);
     mat->entries = NULL;
-    mat->rows = NULL;
 }
 
 void pmlg_nmod_poly_mat_zero(nmod_poly_mat_t mat)
@@ -173,12 +179,20 @@ int pmlg_nmod_poly_mat_equal(const nmod_poly_mat_t a, const nmod_poly_mat_t b)
 
 #define CTX_CACHE 32
 
+/* Contexts cached per modulus, least recently used replaced first.  A call
+ * holds its context from ctx_acquire to ctx_release, so an entry in use is
+ * never replaced; when all CTX_CACHE entries are in use at once, the call
+ * gets a context of its own, destroyed when it is released. */
 static pthread_mutex_t g_cache_mu = PTHREAD_MUTEX_INITIALIZER;
 static struct {
     uint64_t p;
     pmlg_ctx ctx;
+    uint64_t stamp; /* last acquisition (LRU order) */
+    int users;      /* calls holding the context now */
 } g_cache[CTX_CACHE];
 static int g_cache_n = 0;
+static uint64_t g_cache_clock = 0;
+static int g_cache_atexit = 0;
 
 typedef void (*fit_length_fn)(nmod_poly_struct *, slong);
 static fit_length_fn g_fit = NULL;
@@ -211,33 +225,87 @@ static void destroy_cache(void)
     g_cache_n = 0;
 }
 
-static pmlg_ctx ctx_for(uint64_t p, const char *fn)
+static pmlg_ctx new_ctx(uint64_t p, const char *fn)
+{
+    const char *dev = getenv("PMLG_DEVICE");
+    pmlg_ctx c = NULL;
+    if (pmlg_ctx_create(&c, p, dev ? atoi(dev) : 0) != PMLG_OK)
+        die(fn, pmlg_last_error());
+    return c;
+}
+
+static pmlg_ctx ctx_acquire(uint64_t p, const char *fn)
 {
     pthread_mutex_lock(&g_cache_mu);
+    const uint64_t now = ++g_cache_clock;
     for (int i = 0; i < g_cache_n; i++)
         if (g_cache[i].p == p) {
+            g_cache[i].users++;
+            g_cache[i].stamp = now;
             pmlg_ctx c = g_cache[i].ctx;
             pthread_mutex_unlock(&g_cache_mu);
             return c;
         }
-    if (g_cache_n == CTX_CACHE) {
-        pthread_mutex_unlock(&g_cache_mu);
-        die(fn, "more than 32 distinct moduli in one process");
+    int slot = -1;
+    if (g_cache_n < CTX_CACHE) {
+        slot = g_cache_n;
+    } else {
+        /* least recently used idle entry; its destruction synchronises its
+         * stream, so work still queued there completes first */
+        for (int i = 0; i < g_cache_n; i++)
+            if (g_cache[i].users == 0 && (slot < 0 || g_cache[i].stamp < g_cache[slot].stamp))
+                slot = i;
+        if (slot < 0) {
+            pthread_mutex_unlock(&g_cache_mu);
+            return new_ctx(p, fn); /* every entry in use: uncached */
+        }
+        pmlg_ctx_destroy(g_cache[slot].ctx);
+        g_cache[slot].ctx = NULL;
     }
-    const char *dev = getenv("PMLG_DEVICE");
     pmlg_ctx c = NULL;
-    pmlg_status st = pmlg_ctx_create(&c, p, dev ? atoi(dev) : 0);
-    if (st != PMLG_OK) {
+    const char *dev = getenv("PMLG_DEVICE");
+    if (pmlg_ctx_create(&c, p, dev ? atoi(dev) : 0) != PMLG_OK) {
+        if (slot < g_cache_n) { /* the replaced entry is gone: close the gap */
+            g_cache[slot] = g_cache[g_cache_n - 1];
+            g_cache_n--;
+        }
         pthread_mutex_unlock(&g_cache_mu);
         die(fn, pmlg_last_error());
     }
-    if (g_cache_n == 0)
+    if (!g_cache_atexit) {
         atexit(destroy_cache);
-    g_cache[g_cache_n].p = p;
-    g_cache[g_cache_n].ctx = c;
-    g_cache_n++;
+        g_cache_atexit = 1;
+    }
+    g_cache[slot].p = p;
+    g_cache[slot].ctx = c;
+    g_cache[slot].stamp = now;
+    g_cache[slot].users = 1;
+    if (slot == g_cache_n)
+        g_cache_n++;
     pthread_mutex_unlock(&g_cache_mu);
     return c;
+}
+
+static void ctx_release(pmlg_ctx c)
+{
+    pthread_mutex_lock(&g_cache_mu);
+    for (int i = 0; i < g_cache_n; i++)
+        if (g_cache[i].ctx == c) {
+            g_cache[i].users--;
+            pthread_mutex_unlock(&g_cache_mu);
+            return;
+        }
+    pthread_mutex_unlock(&g_cache_mu);
+    pmlg_ctx_destroy(c); /* an uncached context */
+}
+
+/* number of cached contexts (for the drop-in checks) */
+int pmlg_dropin_cached_contexts(void)
+{
+    pthread_mutex_lock(&g_cache_mu);
+    const int n = g_cache_n;
+    pthread_mutex_unlock(&g_cache_mu);
+    return n;
 }
 
 static uint64_t *xalloc(size_t words, const char *fn)
@@ -367,9 +435,11 @@ static void product_entries(nmod_poly_mat_struct *res, const nmod_poly_mat_struc
                 cp[i * n + j] = e->coeffs;
             }
     }
-    check(pmlg_polmat_mulmid_entries(ctx_for(A->modulus, fn), cp, cl, ap, al, bp, bl, m, k, n,
-                                     nlo, nhi, PMLG_METHOD_AUTO),
+    pmlg_ctx ctx = ctx_acquire(A->modulus, fn);
+    check(pmlg_polmat_mulmid_entries(ctx, cp, cl, ap, al, bp, bl, m, k, n, nlo, nhi,
+                                     PMLG_METHOD_AUTO),
           fn);
+    ctx_release(ctx);
     for (slong i = 0; i < m; i++)
         for (slong j = 0; j < n; j++) {
             nmod_poly_struct *e = nmod_poly_mat_entry(res, i, j);
@@ -496,11 +566,12 @@ static void basis_impl(nmod_poly_mat_struct *appbas, slong *shift, const nmod_po
     for (slong i = 0; i < m; i++)
         sh[i] = shift[i];
     int64_t lenP = 0;
-    pmlg_ctx ctx = ctx_for(F->modulus, fn);
+    pmlg_ctx ctx = ctx_acquire(F->modulus, fn);
     if (iterative)
         check(pmlg_mbasis(ctx, P, &lenP, sh, f, m, n, lenF, order, cap), fn);
     else
         check(pmlg_pmbasis(ctx, P, &lenP, sh, f, m, n, lenF, order, cap, PMLG_METHOD_AUTO), fn);
+    ctx_release(ctx);
     free(f);
     for (slong i = 0; i < m; i++)
         shift[i] = (slong)sh[i];
@@ -539,14 +610,6 @@ static uint64_t powmod_h(uint64_t a, uint64_t e, uint64_t p)
     return r;
 }
 
-static uint64_t splitmix64(uint64_t *s)
-{
-    uint64_t z = (*s += 0x9e3779b97f4a7c15ULL);
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-    return z ^ (z >> 31);
-}
-
 static slong entry_degree(const nmod_poly_struct *e)
 {
     slong l = e->length;
@@ -555,12 +618,59 @@ static slong entry_degree(const nmod_poly_struct *e)
     return l - 1;
 }
 
+/* Exact degree of det(P) for a square P (-1 when det = 0): det(P) has degree
+ * at most the sum Dmax of the row degrees, so its values at x = 0, 1, ...,
+ * Dmax (device evaluations and device determinants) fix it; the degree is
+ * the index of the last nonzero Newton divided difference (the Newton basis
+ * polynomial of index j has degree j and leading coefficient 1).  Returns -2
+ * when the points are not distinct mod p (p <= Dmax) or Dmax >= 4096. */
+static slong det_degree(const nmod_poly_mat_struct *P, pmlg_ctx ctx, const char *fn)
+{
+    const slong m = P->r;
+    const uint64_t p = P->modulus;
+    slong dmax = 0;
+    for (slong i = 0; i < m; i++) {
+        slong r = -1;
+        for (slong j = 0; j < m; j++) {
+            const slong d = entry_degree(nmod_poly_mat_entry(P, i, j));
+            r = d > r ? d : r;
+        }
+        if (r < 0)
+            return -1; /* a zero row */
+        dmax += r;
+    }
+    if ((uint64_t)dmax >= p || dmax >= 4096)
+        return -2;
+    const slong len = pmlg_nmod_poly_mat_max_length(P);
+    uint64_t *B = pack(P, len, fn);
+    uint64_t *E = xalloc((size_t)(m * m), fn);
+    uint64_t *a = xalloc((size_t)(dmax + 1), fn);
+    for (slong x = 0; x <= dmax; x++) {
+        int64_t rk = 0;
+        check(pmlg_polmat_eval(ctx, E, B, m, m, len, (uint64_t)x), fn);
+        check(pmlg_const_rank_det(ctx, E, m, m, &rk, &a[x]), fn);
+    }
+    /* divided differences at the points 0..dmax: x_i - x_{i-j} = j */
+    for (slong j = 1; j <= dmax; j++) {
+        const uint64_t ij = powmod_h((uint64_t)j, p - 2, p);
+        for (slong i = dmax; i >= j; i--)
+            a[i] = mulmod_h(a[i] >= a[i - 1] ? a[i] - a[i - 1] : a[i] + (p - a[i - 1]), ij, p);
+    }
+    slong deg = -1;
+    for (slong j = 0; j <= dmax; j++)
+        if (a[j])
+            deg = j;
+    free(a);
+    free(E);
+    free(B);
+    return deg;
+}
+
 /* Returns 1 when appbas is a shift-ordered weak Popov approximant basis of F
  * at `order` (ROW_LOWER), else 0, printing which test failed, as the
  * reference does.  The residual appbas*F mod x^(order+1) is a device middle
- * product; det(appbas) = lambda x^D (D the sum of the diagonal degrees) is
- * checked at x = 1 and three seeded points by device evaluations and device
- * determinants; the rank of [appbas(0) | coefficient `order` of appbas*F] is a
+ * product; the determinant's degree is tested as the reference tests it (see
+ * below); the rank of [appbas(0) | coefficient `order` of appbas*F] is a
  * device elimination. */
 int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_poly_mat_t F,
                                        slong order, const slong *shift, orientation_t orient)
@@ -595,10 +705,11 @@ int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_
         }
         prev = piv;
     }
+    const int popov_ok = success;
     if (m == 0)
         return success;
 
-    pmlg_ctx ctx = ctx_for(p, fn);
+    pmlg_ctx ctx = ctx_acquire(p, fn);
     slong lenP = pmlg_nmod_poly_mat_max_length(appbas);
     slong lenF = pmlg_nmod_poly_mat_max_length(F);
     const slong lenR = order + 1;
@@ -624,31 +735,22 @@ int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_
                 break;
             }
 
-    /* det(appbas) = lambda x^D */
+    /* deg det(appbas) == D, D the sum of the diagonal degrees
+     * (verification.c:73-84 tests the degree only).  A square shifted ordered
+     * weak Popov matrix has row i's pivot on the diagonal, so its shifted
+     * leading matrix is lower triangular with a nonzero diagonal: it is
+     * shift-reduced and deg det = sum_i (deg P_ii + s_i) - sum_i s_i = D.  The
+     * determinant is therefore computed only when the weak Popov test failed
+     * (the result is already 0 then; only this message depends on it). */
     slong D = 0;
     for (slong i = 0; i < m; i++)
         D += entry_degree(nmod_poly_mat_entry(appbas, i, i));
-    {
-        uint64_t *P = pack(appbas, lenP, fn);
-        uint64_t *E = xalloc((size_t)(m * m), fn);
-        int64_t rk = 0;
-        uint64_t lambda = 0, dv = 0;
-        check(pmlg_polmat_eval(ctx, E, P, m, m, lenP, 1 % p), fn);
-        check(pmlg_const_rank_det(ctx, E, m, m, &rk, &lambda), fn);
-        int ok = lambda != 0 && D >= 0;
-        uint64_t st = 0x5eed5eed5eed5eedULL;
-        for (int t = 0; t < 3 && ok; t++) {
-            const uint64_t x = p > 2 ? 2 + splitmix64(&st) % (p - 2) : 1 % p;
-            check(pmlg_polmat_eval(ctx, E, P, m, m, lenP, x), fn);
-            check(pmlg_const_rank_det(ctx, E, m, m, &rk, &dv), fn);
-            ok = dv == mulmod_h(lambda, powmod_h(x, (uint64_t)D, p), p);
-        }
-        if (!ok) {
+    if (!popov_ok) {
+        const slong dd = det_degree(appbas, ctx, fn);
+        if (dd != -2 && dd != D) {
             printf("determinant is not lambda * x**(sum(diag-deg))\n");
             success = 0;
         }
-        free(E);
-        free(P);
     }
 
     /* [appbas(0) | C] of full rank, C = coefficient `order` of the residual */
@@ -671,6 +773,7 @@ int nmod_poly_mat_is_approximant_basis(const nmod_poly_mat_t appbas, const nmod_
         free(CP);
     }
     free(R);
+    ctx_release(ctx);
     return success;
 }
 
@@ -721,7 +824,7 @@ static uint64_t *kernel_device(const uint64_t *g, slong dim, slong other, slong 
                                int64_t *sh, int64_t *piv, int algo, int64_t *nul, int64_t *cap,
                                const char *fn)
 {
-    pmlg_ctx ctx = ctx_for(p, fn);
+    pmlg_ctx ctx = ctx_acquire(p, fn);
     int64_t amp = 0, mn = dim ? sh[0] : 0, mx = mn;
     for (slong i = 0; i < dim; i++) {
         mn = sh[i] < mn ? sh[i] : mn;
@@ -746,6 +849,7 @@ static uint64_t *kernel_device(const uint64_t *g, slong dim, slong other, slong 
         }
         check(st, fn);
         free(sh0);
+        ctx_release(ctx);
         *cap = c;
         return K;
     }
@@ -922,7 +1026,7 @@ static slong polmat_rank(const nmod_poly_mat_struct *M, const char *fn)
     const uint64_t p = M->modulus;
     if ((uint64_t)(full * d) >= p)
         die(fn, "modulus too small for the exact rank by evaluation");
-    pmlg_ctx ctx = ctx_for(p, fn);
+    pmlg_ctx ctx = ctx_acquire(p, fn);
     const slong len = d + 1;
     uint64_t *P = pack(M, len, fn);
     uint64_t *E = xalloc((size_t)(r * c), fn);
@@ -935,6 +1039,7 @@ static slong polmat_rank(const nmod_poly_mat_struct *M, const char *fn)
     }
     free(E);
     free(P);
+    ctx_release(ctx);
     return best;
 }
 

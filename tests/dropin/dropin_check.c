@@ -228,6 +228,31 @@ static void check_basis(int iterative, slong m, slong n, slong lenF, slong order
     }
     if (nmod_poly_mat_is_approximant_basis(P, F, order, shift0, ROW_LOWER))
         fail("is_approximant_basis accepts x * row 0");
+    /* (x + 1) * row 0 instead: approximants, weak Popov, deg det = D and
+     * [P(0) | C] unchanged, so the reference's certificate, whose determinant
+     * test checks the degree only (verification.c:73-84), accepts it */
+    for (slong j = 0; j < m; j++) {
+        e = nmod_poly_mat_entry(P, 0, j);
+        for (slong d = 0; d + 1 < e->length; d++)
+            e->coeffs[d] = (e->coeffs[d] + e->coeffs[d + 1]) % p;
+    }
+    if (!nmod_poly_mat_is_approximant_basis(P, F, order, shift0, ROW_LOWER))
+        fail("is_approximant_basis rejects (x + 1) * row 0, which the reference accepts");
+    if (m >= 2) {
+        /* rows 0 and 1 exchanged: no longer ordered weak Popov */
+        for (slong j = 0; j < m; j++) {
+            nmod_poly_struct t = *nmod_poly_mat_entry(P, 0, j);
+            *nmod_poly_mat_entry(P, 0, j) = *nmod_poly_mat_entry(P, 1, j);
+            *nmod_poly_mat_entry(P, 1, j) = t;
+        }
+        if (nmod_poly_mat_is_approximant_basis(P, F, order, shift0, ROW_LOWER))
+            fail("is_approximant_basis accepts a basis with two rows exchanged");
+        for (slong j = 0; j < m; j++) {
+            nmod_poly_struct t = *nmod_poly_mat_entry(P, 0, j);
+            *nmod_poly_mat_entry(P, 0, j) = *nmod_poly_mat_entry(P, 1, j);
+            *nmod_poly_mat_entry(P, 1, j) = t;
+        }
+    }
     /* a changed low coefficient breaks the residual */
     e = nmod_poly_mat_entry(P, m - 1, m - 1);
     if (e->length == 0) {
@@ -373,6 +398,45 @@ static void check_kernel(slong m, slong n, slong len, slong r, uint64_t p)
     nmod_poly_mat_clear(F);
 }
 
+/* More distinct moduli than the drop-in caches contexts for (32), as the
+ * reference's tests draw a new random prime per iteration
+ * (t-mul_geometric.c:59-60, t-pmbasis.c:468-476): each product is still
+ * exact, and the cache stays at its bound. */
+static void check_many_moduli(void)
+{
+    int done = 0;
+    for (int i = 0; i < 40; i++) {
+        uint64_t p = ((uint64_t)1 << (12 + i)) + 1000003ULL * (uint64_t)i;
+        while (!orc_is_prime(p))
+            p++;
+        nmod_poly_mat_t A, B, C;
+        nmod_poly_mat_init(A, 2, 3, p);
+        nmod_poly_mat_init(B, 3, 2, p);
+        nmod_poly_mat_init(C, 2, 2, p);
+        rand_mat(A, 20, 1, 0);
+        rand_mat(B, 20, 1, 0);
+        nmod_poly_mat_multiply(C, A, B);
+        const slong la = nmod_poly_mat_max_length(A), lb = nmod_poly_mat_max_length(B);
+        const slong lc = (la && lb) ? la + lb - 1 : 1;
+        uint64_t *a = pack(A, la ? la : 1), *b = pack(B, lb ? lb : 1);
+        uint64_t *c = calloc((size_t)(4 * lc + 1), sizeof(uint64_t));
+        if (la && lb)
+            orc_polmat_mul(c, a, b, 2, 3, 2, la, lb, p);
+        expect_equal(C, c, lc, "multiply over many moduli");
+        free(a);
+        free(b);
+        free(c);
+        nmod_poly_mat_clear(A);
+        nmod_poly_mat_clear(B);
+        nmod_poly_mat_clear(C);
+        done++;
+    }
+    if (pmlg_dropin_cached_contexts() > 32)
+        fail("the drop-in keeps more than 32 contexts");
+    printf("ok multiply over %d distinct moduli (%d contexts cached)\n", done,
+           pmlg_dropin_cached_contexts());
+}
+
 int main(void)
 {
     check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 5, 7, 3, 300, 1, 1, P60);
@@ -403,6 +467,7 @@ int main(void)
     check_kernel(5, 5, 6, 3, P30);
     check_kernel(9, 3, 40, 3, P60);
     check_kernel(4, 6, 3, 1, P_NOFFT);
+    check_many_moduli();
     printf("all drop-in checks passed\n");
     return 0;
 }

@@ -240,6 +240,134 @@ def test_certificate_rejects_planted_mistakes():
     assert O.is_approximant_basis(Q, F, order, shift, p) == 0
 
 
+def _pmul(a, b, p):
+    if not a or not b:
+        return []
+    c = [0] * (len(a) + len(b) - 1)
+    for i, x in enumerate(a):
+        for j, y in enumerate(b):
+            c[i + j] = (c[i + j] + x * y) % p
+    return c
+
+
+def _padd(a, b, p, sign=1):
+    c = [0] * max(len(a), len(b))
+    for i, x in enumerate(a):
+        c[i] = x
+    for i, y in enumerate(b):
+        c[i] = (c[i] + sign * y) % p
+    while c and c[-1] == 0:
+        c.pop()
+    return c
+
+
+def _det_degree(P, p):
+    """deg det(P) of a small square polynomial matrix (Leibniz formula over
+    Python integers, -1 for det = 0): independent of the oracle."""
+    m = P.shape[0]
+    ent = [[[int(v) for v in P[i, j]] for j in range(m)] for i in range(m)]
+    det = []
+    for perm in itertools.permutations(range(m)):
+        sgn = 1
+        for i in range(m):
+            for j in range(i + 1, m):
+                if perm[i] > perm[j]:
+                    sgn = -sgn
+        term = [1]
+        for i in range(m):
+            term = _pmul(term, ent[i][perm[i]], p)
+        det = _padd(det, term, p, sgn)
+    return len(det) - 1
+
+
+def _deg(v):
+    nz = np.nonzero(v)[0]
+    return int(nz[-1]) if len(nz) else -1
+
+
+def _is_ord_weak_popov(P, shift):
+    """pivot (rightmost entry reaching the shifted row degree) strictly
+    increasing along the rows, no zero row"""
+    prev = -1
+    for i in range(P.shape[0]):
+        piv, mx = -1, 0
+        for j in range(P.shape[1]):
+            d = _deg(P[i, j])
+            if d >= 0 and (piv == -1 or mx <= d + shift[j]):
+                piv, mx = j, d + shift[j]
+        if piv == -1 or piv <= prev:
+            return False
+        prev = piv
+    return True
+
+
+def test_certificate_determinant_degree_is_the_references_test():
+    """Bit 4 is set exactly when deg det(P) differs from the sum D of the
+    diagonal degrees (verification.c:73-84), with deg det from an independent
+    Leibniz expansion.  Random 3x3 and 4x4 matrices against a zero F (so only
+    the form and determinant tests can fail); for the weak Popov ones the
+    expansion also confirms deg det = D, the fact the oracle uses for them."""
+    rng = np.random.default_rng(21)
+    p = 998244353
+    seen = {True: 0, False: 0}
+    popov_seen = 0
+    for trial in range(120):
+        m = 3 + trial % 2
+        L = 6
+        P = rand(rng, (m, m, L), p)
+        for i in range(m):
+            for j in range(m):
+                d = int(rng.integers(-1, L))
+                P[i, j, d + 1:] = 0
+                if d >= 0 and P[i, j, d] == 0:
+                    P[i, j, d] = 1
+        if trial % 3 == 0:  # make some of them ordered weak Popov (diagonal pivots)
+            for i in range(m):
+                di = int(rng.integers(0, L))
+                P[i, i, :] = 0
+                P[i, i, :di + 1] = rand(rng, di + 1, p)
+                P[i, i, di] = 1 + P[i, i, di] % (p - 1)
+                for j in range(m):
+                    if j > i:
+                        P[i, j, di:] = 0
+                    elif j < i:
+                        P[i, j, di + 1:] = 0
+        shift = np.zeros(m, dtype=np.int64)
+        F = np.zeros((m, 2, 3), dtype=np.uint64)
+        D = sum(_deg(P[i, i]) for i in range(m))
+        dd = _det_degree(P, p)
+        popov = _is_ord_weak_popov(P, shift)
+        got = O.is_approximant_basis(P, F, 2, shift, p)
+        if popov:
+            popov_seen += 1
+            assert dd == D
+            assert not got & 4
+        else:
+            assert bool(got & 4) == (dd != D), (trial, dd, D)
+            seen[dd != D] += 1
+    assert popov_seen >= 20 and seen[True] >= 10 and seen[False] >= 5, (popov_seen, seen)
+
+
+def test_certificate_accepts_what_the_references_accepts():
+    """Multiplying row 0 of a basis by (x + 1) keeps approximants, the weak
+    Popov form, deg det = D and the rank of [P(0) | C]: the reference's
+    certificate (a degree test, not a monomial test) accepts it, and so does
+    this one; multiplying by x loses the rank of [P(0) | C] and is rejected."""
+    rng = np.random.default_rng(4)
+    p = 998244353
+    m, n, order = 5, 2, 24
+    F = rand(rng, (m, n, order), p)
+    shift = np.zeros(m, dtype=np.int64)
+    P, _ = O.pmbasis(F, order, shift, p)
+    Q = np.concatenate([P, np.zeros((m, m, 1), dtype=np.uint64)], axis=2).astype(object)
+    Q[0, :, 1:] = (Q[0, :, 1:] + Q[0, :, :-1]) % p
+    assert O.is_approximant_basis(Q.astype(np.uint64), F, order, shift, p) == 0
+    X = np.concatenate([P, np.zeros((m, m, 1), dtype=np.uint64)], axis=2)
+    X[0, :, 1:] = X[0, :, :-1].copy()
+    X[0, :, 0] = 0
+    assert O.is_approximant_basis(X, F, order, shift, p) == 8
+
+
 SHIFTS = {
     "uniform": lambda m, d: np.zeros(m),
     "increasing": lambda m, d: np.arange(m),
