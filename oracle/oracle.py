@@ -71,6 +71,10 @@ def lib():
                                                _i64p, u64, u64]
         L.orc_free.restype = None
         L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [c_int]
+        L.orc_threads.restype = c_int
+        L.orc_threads.argtypes = []
         _LIB = L
     return _LIB
 
@@ -85,6 +89,16 @@ def _pi(a):
 
 def _c(a):
     return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def set_threads(n):
+    """Host threads of the oracle's product, transform and evaluation loops
+    (n <= 0: OpenMP's default, OMP_NUM_THREADS or all cores)."""
+    lib().orc_set_threads(int(n))
+
+
+def threads():
+    return lib().orc_threads()
 
 
 def mulmod(a, b, p):

@@ -8,8 +8,12 @@
  */
 #include "pml_oracle.h"
 
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef unsigned __int128 u128;
 
@@ -227,17 +231,23 @@ void orc_polmat_mul_school(uint64_t *C, const uint64_t *A, const uint64_t *B,
             }
 }
 
-/* --- textbook NTT (bit reversal + iterative Cooley-Tukey), Shoup products --- */
+/* --- radix-2 NTT: decimation in frequency forward (natural order in,
+ * bit-reversed order out), decimation in time inverse (bit-reversed in,
+ * natural out), so no permutation pass is needed between them; pointwise
+ * products work position by position.  Shoup products by the twiddles
+ * (Harvey's lazy butterflies, values in [0, 2p), for p < 2^62; reduced after
+ * every butterfly above that). --- */
 
 typedef struct
 {
     int64_t N;
     int logn;
     uint64_t p;
-    uint64_t *w;  /* w[j] = root^j, j < N/2 */
-    uint64_t *wq; /* floor(w[j] * 2^64 / p) */
-    uint64_t *wi;  /* inverse root powers */
-    uint64_t *wiq;
+    int lazy;     /* 4p < 2^64 */
+    /* per-stage tables: the stage with half-size h uses w^(j N/(2h)), j < h,
+     * stored contiguously from offset N - 2h (h = N/2 at 0, ..., h = 1 at N-2) */
+    uint64_t *w, *wq;   /* powers of the root */
+    uint64_t *wi, *wiq; /* powers of its inverse */
     uint64_t ninv, ninvq;
 } ntt_plan;
 
@@ -246,11 +256,17 @@ static inline uint64_t shoup_pre(uint64_t w, uint64_t p)
     return (uint64_t)(((u128)w << 64) / p);
 }
 
+/* x * w mod p in [0, 2p) for any x < 2^64 and w < p < 2^63 */
+static inline uint64_t shoup_lazy(uint64_t x, uint64_t w, uint64_t wq, uint64_t p)
+{
+    uint64_t q = (uint64_t)(((u128)x * wq) >> 64);
+    return x * w - q * p;
+}
+
 /* x * w mod p, for x < 2^64, w < p < 2^63; exact result in [0,p) */
 static inline uint64_t shoup_mul(uint64_t x, uint64_t w, uint64_t wq, uint64_t p)
 {
-    uint64_t q = (uint64_t)(((u128)x * wq) >> 64);
-    uint64_t r = x * w - q * p;
+    uint64_t r = shoup_lazy(x, w, wq, p);
     return r >= p ? r - p : r;
 }
 
@@ -264,23 +280,30 @@ static int ntt_plan_init(ntt_plan *P, int logn, uint64_t p)
     P->N = (int64_t)1 << logn;
     P->logn = logn;
     P->p = p;
-    int64_t h = P->N / 2 > 0 ? P->N / 2 : 1;
-    P->w = malloc(h * sizeof(uint64_t));
-    P->wq = malloc(h * sizeof(uint64_t));
-    P->wi = malloc(h * sizeof(uint64_t));
-    P->wiq = malloc(h * sizeof(uint64_t));
-    uint64_t rinv = logn > 0 ? orc_invmod(root, p) : 1;
-    uint64_t x = 1 % p, y = 1 % p;
-    for (int64_t j = 0; j < h; j++)
+    P->lazy = p < ((uint64_t)1 << 62);
+    const int64_t N = P->N, sz = N > 1 ? N : 1;
+    P->w = malloc(sz * sizeof(uint64_t));
+    P->wq = malloc(sz * sizeof(uint64_t));
+    P->wi = malloc(sz * sizeof(uint64_t));
+    P->wiq = malloc(sz * sizeof(uint64_t));
+    const uint64_t rinv = logn > 0 ? orc_invmod(root, p) : 1;
+    /* the stage with half-size h uses the (2h)-th root root^(N/2h) */
+    for (int64_t h = N / 2; h >= 1; h >>= 1)
     {
-        P->w[j] = x;
-        P->wq[j] = shoup_pre(x, p);
-        P->wi[j] = y;
-        P->wiq[j] = shoup_pre(y, p);
-        x = orc_mulmod(x, root, p);
-        y = orc_mulmod(y, rinv, p);
+        const uint64_t st = orc_powmod(root, (uint64_t)(N / (2 * h)), p);
+        const uint64_t sti = orc_powmod(rinv, (uint64_t)(N / (2 * h)), p);
+        uint64_t x = 1 % p, y = 1 % p;
+        for (int64_t j = 0; j < h; j++)
+        {
+            P->w[N - 2 * h + j] = x;
+            P->wq[N - 2 * h + j] = shoup_pre(x, p);
+            P->wi[N - 2 * h + j] = y;
+            P->wiq[N - 2 * h + j] = shoup_pre(y, p);
+            x = orc_mulmod(x, st, p);
+            y = orc_mulmod(y, sti, p);
+        }
     }
-    P->ninv = orc_invmod((uint64_t)(P->N % p), p);
+    P->ninv = orc_invmod((uint64_t)(N % p), p);
     P->ninvq = shoup_pre(P->ninv, p);
     return 0;
 }
@@ -293,47 +316,101 @@ static void ntt_plan_clear(ntt_plan *P)
     free(P->wiq);
 }
 
-static void ntt_core(uint64_t *a, const ntt_plan *P, const uint64_t *w, const uint64_t *wq)
-{
-    const int64_t N = P->N;
-    const uint64_t p = P->p;
-    for (int64_t i = 1, j = 0; i < N; i++)
-    {
-        int64_t bit = N >> 1;
-        for (; j & bit; bit >>= 1)
-            j ^= bit;
-        j ^= bit;
-        if (i < j)
-        {
-            uint64_t t = a[i];
-            a[i] = a[j];
-            a[j] = t;
-        }
-    }
-    for (int64_t len = 2; len <= N; len <<= 1)
-    {
-        const int64_t half = len >> 1, step = N / len;
-        for (int64_t i = 0; i < N; i += len)
-            for (int64_t j = 0; j < half; j++)
-            {
-                uint64_t u = a[i + j];
-                uint64_t v = shoup_mul(a[i + j + half], w[j * step], wq[j * step], p);
-                a[i + j] = addmod(u, v, p);
-                a[i + j + half] = submod(u, v, p);
-            }
-    }
-}
-
+/* forward DIF: a[] in [0, p) natural order -> evaluations in [0, p) at the
+ * N-th roots, position t holding the value at root^bitrev(t) */
 static void ntt_forward(uint64_t *a, const ntt_plan *P)
 {
-    ntt_core(a, P, P->w, P->wq);
+    const int64_t N = P->N;
+    const uint64_t p = P->p, p2 = 2 * p;
+    for (int64_t h = N / 2; h >= 1; h >>= 1)
+    {
+        const uint64_t *w = P->w + (N - 2 * h), *wq = P->wq + (N - 2 * h);
+        for (int64_t i = 0; i < N; i += 2 * h)
+        {
+            uint64_t *x = a + i, *y = a + i + h;
+            if (P->lazy)
+                for (int64_t j = 0; j < h; j++)
+                {
+                    const uint64_t u = x[j], v = y[j]; /* [0, 2p) */
+                    const uint64_t s = u + v;
+                    x[j] = s >= p2 ? s - p2 : s;
+                    y[j] = shoup_lazy(u + p2 - v, w[j], wq[j], p);
+                }
+            else
+                for (int64_t j = 0; j < h; j++)
+                {
+                    const uint64_t u = x[j], v = y[j];
+                    x[j] = addmod(u, v, p);
+                    y[j] = shoup_mul(submod(u, v, p), w[j], wq[j], p);
+                }
+        }
+    }
+    if (P->lazy)
+        for (int64_t i = 0; i < N; i++)
+            a[i] = a[i] >= p ? a[i] - p : a[i];
 }
 
+/* inverse DIT: bit-reversed evaluations in [0, p) -> N^-1 * transform at
+ * the inverse root, natural order, in [0, p): the exact inverse of
+ * ntt_forward */
 static void ntt_inverse(uint64_t *a, const ntt_plan *P)
 {
-    ntt_core(a, P, P->wi, P->wiq);
-    for (int64_t i = 0; i < P->N; i++)
-        a[i] = shoup_mul(a[i], P->ninv, P->ninvq, P->p);
+    const int64_t N = P->N;
+    const uint64_t p = P->p, p2 = 2 * p;
+    for (int64_t h = 1; h < N; h <<= 1)
+    {
+        const uint64_t *w = P->wi + (N - 2 * h), *wq = P->wiq + (N - 2 * h);
+        for (int64_t i = 0; i < N; i += 2 * h)
+        {
+            uint64_t *x = a + i, *y = a + i + h;
+            if (P->lazy)
+                for (int64_t j = 0; j < h; j++)
+                {
+                    uint64_t u = x[j]; /* [0, 2p) */
+                    const uint64_t v = shoup_lazy(y[j], w[j], wq[j], p);
+                    const uint64_t s = u + v, d = u + p2 - v; /* [0, 4p) */
+                    x[j] = s >= p2 ? s - p2 : s;
+                    y[j] = d >= p2 ? d - p2 : d;
+                }
+            else
+                for (int64_t j = 0; j < h; j++)
+                {
+                    const uint64_t u = x[j], v = shoup_mul(y[j], w[j], wq[j], p);
+                    x[j] = addmod(u, v, p);
+                    y[j] = submod(u, v, p);
+                }
+        }
+    }
+    for (int64_t i = 0; i < N; i++)
+        a[i] = shoup_mul(a[i], P->ninv, P->ninvq, p);
+}
+
+/* ---------------- threads (OpenMP) ---------------- */
+static int g_threads = 0; /* 0: OpenMP's default */
+
+void orc_set_threads(int n)
+{
+    g_threads = n > 0 ? n : 0;
+}
+
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Products of residues folded into 128 bits: (p-1)^2 per term, so `fold`
+ * terms fit before a reduction (for p < 2^60 that is >= 256). */
+static inline int64_t fold_terms(uint64_t p)
+{
+    const u128 sq = (u128)(p - 1) * (p - 1);
+    if (sq == 0)
+        return INT64_MAX;
+    const u128 f = (~(u128)0 - (u128)(p - 1)) / sq;
+    return f > (u128)INT64_MAX ? INT64_MAX : (int64_t)f;
 }
 
 int orc_polmat_mul_ntt(uint64_t *C, const uint64_t *A, const uint64_t *B,
@@ -343,43 +420,97 @@ int orc_polmat_mul_ntt(uint64_t *C, const uint64_t *A, const uint64_t *B,
     if (lenA <= 0 || lenB <= 0)
         return 0;
     const int64_t lenC = lenA + lenB - 1;
-    int logn = 0;
+    int logn = 2; /* at least 4 points: the pointwise loop takes four at a time */
     while (((int64_t)1 << logn) < lenC)
         logn++;
     ntt_plan P;
     if (ntt_plan_init(&P, logn, p))
         return -1;
     const int64_t N = P.N;
-    uint64_t *Ah = calloc((size_t)(m * k) * N, sizeof(uint64_t));
-    uint64_t *Bh = calloc((size_t)(k * n) * N, sizeof(uint64_t));
-    uint64_t *t = malloc(N * sizeof(uint64_t));
-    for (int64_t e = 0; e < m * k; e++)
+    const int nt = orc_threads();
+    uint64_t *Ah = malloc((size_t)(m * k) * N * sizeof(uint64_t) + 8);
+    uint64_t *Bh = malloc((size_t)(k * n) * N * sizeof(uint64_t) + 8);
+    uint64_t *Ch = malloc((size_t)(m * n) * N * sizeof(uint64_t) + 8);
+    /* forward transforms, one entry per task */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int64_t e = 0; e < m * k + k * n; e++)
     {
-        memcpy(Ah + e * N, A + e * lenA, lenA * sizeof(uint64_t));
-        ntt_forward(Ah + e * N, &P);
+        const int a = e < m * k;
+        uint64_t *d = a ? Ah + e * N : Bh + (e - m * k) * N;
+        const uint64_t *src = a ? A + e * lenA : B + (e - m * k) * lenB;
+        const int64_t l = a ? lenA : lenB;
+        memcpy(d, src, (size_t)l * sizeof(uint64_t));
+        memset(d + l, 0, (size_t)(N - l) * sizeof(uint64_t));
+        ntt_forward(d, &P);
     }
-    for (int64_t e = 0; e < k * n; e++)
+    /* pointwise m x k by k x n products on blocks of SB points: each task
+     * copies its block of every operand entry into contiguous panels
+     * (A as [i][kk][s], B as [j][kk][s], about 1 MiB), then forms exact
+     * 128-bit sums of four points at a time in registers, folded every `fold`
+     * terms */
+    int64_t SB = 64;
+    while (SB > 4 && (m * k + k * n) * SB * 8 > (1 << 20))
+        SB >>= 1;
+    if (SB > N)
+        SB = N;
+    const int64_t nsb = N / SB;
+    const int64_t fold = fold_terms(p);
+#pragma omp parallel num_threads(nt)
     {
-        memcpy(Bh + e * N, B + e * lenB, lenB * sizeof(uint64_t));
-        ntt_forward(Bh + e * N, &P);
-    }
-    for (int64_t i = 0; i < m; i++)
-        for (int64_t j = 0; j < n; j++)
+        uint64_t *pa = malloc((size_t)(m * k * SB) * sizeof(uint64_t) + 8);
+        uint64_t *pb = malloc((size_t)(k * n * SB) * sizeof(uint64_t) + 8);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t blk = 0; blk < nsb; blk++)
         {
-            for (int64_t s = 0; s < N; s++)
-            {
-                acc128 acc;
-                acc_init(&acc, p);
-                for (int64_t kk = 0; kk < k; kk++)
-                    acc_mac(&acc, Ah[(i * k + kk) * N + s], Bh[(kk * n + j) * N + s]);
-                t[s] = acc_get(&acc);
-            }
-            ntt_inverse(t, &P);
-            memcpy(C + (i * n + j) * lenC, t, lenC * sizeof(uint64_t));
+            const int64_t s0 = blk * SB;
+            for (int64_t e = 0; e < m * k; e++)
+                memcpy(pa + e * SB, Ah + e * N + s0, (size_t)SB * sizeof(uint64_t));
+            for (int64_t kk = 0; kk < k; kk++)
+                for (int64_t j = 0; j < n; j++)
+                    memcpy(pb + (j * k + kk) * SB, Bh + (kk * n + j) * N + s0,
+                           (size_t)SB * sizeof(uint64_t));
+            for (int64_t i = 0; i < m; i++)
+                for (int64_t j = 0; j < n; j++)
+                {
+                    uint64_t *out = Ch + (i * n + j) * N + s0;
+                    for (int64_t s = 0; s < SB; s += 4)
+                    {
+                        const uint64_t *a = pa + i * k * SB + s, *b = pb + j * k * SB + s;
+                        u128 c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+                        for (int64_t kk = 0; kk < k; kk++, a += SB, b += SB)
+                        {
+                            c0 += (u128)a[0] * b[0];
+                            c1 += (u128)a[1] * b[1];
+                            c2 += (u128)a[2] * b[2];
+                            c3 += (u128)a[3] * b[3];
+                            if ((kk + 1) % fold == 0)
+                            {
+                                c0 %= p;
+                                c1 %= p;
+                                c2 %= p;
+                                c3 %= p;
+                            }
+                        }
+                        out[s] = (uint64_t)(c0 % p);
+                        out[s + 1] = (uint64_t)(c1 % p);
+                        out[s + 2] = (uint64_t)(c2 % p);
+                        out[s + 3] = (uint64_t)(c3 % p);
+                    }
+                }
         }
-    free(t);
+        free(pa);
+        free(pb);
+    }
+    /* inverse transforms, one entry per task */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int64_t e = 0; e < m * n; e++)
+    {
+        ntt_inverse(Ch + e * N, &P);
+        memcpy(C + e * lenC, Ch + e * N, (size_t)lenC * sizeof(uint64_t));
+    }
     free(Ah);
     free(Bh);
+    free(Ch);
     ntt_plan_clear(&P);
     return 0;
 }
@@ -434,11 +565,23 @@ void orc_polmat_mulmid(uint64_t *out, const uint64_t *A, const uint64_t *B,
 void orc_polmat_eval(uint64_t *out, const uint64_t *A, int64_t r, int64_t c, int64_t len,
                      uint64_t x, uint64_t p)
 {
+    x %= p;
+    /* Horner with Shoup products by the fixed point (p < 2^63), else exact
+     * 128-bit remainders; one entry per task */
+    const int shoup = p < ((uint64_t)1 << 63);
+    const uint64_t xq = shoup ? shoup_pre(x, p) : 0;
+    const int nt = r * c * len >= (1 << 16) ? orc_threads() : 1;
+#pragma omp parallel for schedule(static) num_threads(nt)
     for (int64_t e = 0; e < r * c; e++)
     {
         uint64_t v = 0;
-        for (int64_t d = len - 1; d >= 0; d--)
-            v = addmod(orc_mulmod(v, x, p), A[e * len + d], p);
+        const uint64_t *a = A + e * len;
+        if (shoup)
+            for (int64_t d = len - 1; d >= 0; d--)
+                v = addmod(shoup_mul(v, x, xq, p), a[d], p);
+        else
+            for (int64_t d = len - 1; d >= 0; d--)
+                v = addmod(orc_mulmod(v, x, p), a[d], p);
         out[e] = v;
     }
 }

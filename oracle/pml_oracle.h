@@ -47,6 +47,12 @@ int orc_two_adic_valuation(uint64_t n);                 /* v_2(n), n > 0 */
  * yields order exactly 2^k. Returns 0 if 2^k does not divide p-1. */
 uint64_t orc_root_of_unity_2k(uint64_t p, int k);
 
+/* ---------------- threads ---------------- */
+/* Host threads of the product, evaluation and transform loops (OpenMP);
+ * n <= 0 restores OpenMP's default (OMP_NUM_THREADS, else all cores). */
+void orc_set_threads(int n);
+int orc_threads(void);
+
 /* ---------------- seeded generator ---------------- */
 /* SplitMix64 stream; uniform residues in [0,p) by rejection on the top
  * bitlen(p) bits. */
@@ -59,8 +65,10 @@ void orc_rand_uniform(uint64_t *out, int64_t n, uint64_t p, uint64_t seed);
 void orc_polmat_mul_school(uint64_t *C, const uint64_t *A, const uint64_t *B,
                            int64_t m, int64_t k, int64_t n,
                            int64_t lenA, int64_t lenB, uint64_t p);
-/* Same result through a textbook radix-2 NTT (needs 2^ceil(log2(lenA+lenB-1))
- * | p-1 and p < 2^63); returns 0 on success, -1 if the prime does not allow it. */
+/* Same result through a radix-2 NTT (needs 2^ceil(log2(lenA+lenB-1)) | p-1
+ * and p < 2^63): decimation in frequency forward, decimation in time inverse,
+ * exact 128-bit pointwise sums; returns 0 on success, -1 if the prime does not
+ * allow it. */
 int orc_polmat_mul_ntt(uint64_t *C, const uint64_t *A, const uint64_t *B,
                        int64_t m, int64_t k, int64_t n,
                        int64_t lenA, int64_t lenB, uint64_t p);

@@ -100,11 +100,28 @@ def test_school_extreme_values():
             assert int(C[0, 0, d]) == cnt % p
 
 
-@pytest.mark.parametrize("p", FFT_PRIMES)
+@pytest.mark.parametrize("p", FFT_PRIMES + [4611686018429485057])
 def test_ntt_matches_school(p):
+    """The NTT product (lazy butterflies below 2^62, reduced ones above;
+    pointwise panels of 64, 32, ... points; a fold of the 128-bit sums every
+    few terms for primes above 2^62) equals the closed form, with one thread
+    and with several."""
     rng = np.random.default_rng(p % 997)
-    for (m, k, n, la, lb) in [(1, 1, 1, 1, 1), (3, 4, 2, 33, 31), (2, 2, 2, 200, 57), (1, 3, 1, 512, 513)]:
+    for (m, k, n, la, lb) in [(1, 1, 1, 1, 1), (3, 4, 2, 33, 31), (2, 2, 2, 200, 57),
+                              (1, 3, 1, 512, 513), (2, 9, 3, 40, 3), (40, 40, 41, 4, 3)]:
         A, B = rand(rng, (m, k, la), p), rand(rng, (k, n, lb), p)
+        ref = O.polmat_mul(A, B, p, "school")
+        for th in (1, 3):
+            O.set_threads(th)
+            assert np.array_equal(O.polmat_mul(A, B, p, "ntt"), ref), (m, k, n, la, lb, th)
+    O.set_threads(0)
+
+
+def test_ntt_extreme_values():
+    """All residues p-1: the largest 128-bit sums the pointwise loop folds."""
+    for p in (1152921504556515329, 4611686018429485057, 998244353):
+        A = np.full((2, 70, 30), p - 1, dtype=np.uint64)
+        B = np.full((70, 3, 20), p - 1, dtype=np.uint64)
         assert np.array_equal(O.polmat_mul(A, B, p, "ntt"), O.polmat_mul(A, B, p, "school"))
 
 
