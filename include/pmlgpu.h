@@ -85,6 +85,29 @@ pmlg_status pmlg_polmat_mul_dev(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA,
                                 const uint64_t *dB, int64_t m, int64_t k, int64_t n,
                                 int64_t lenA, int64_t lenB, int method, void *stream);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same product as pmlg_polmat_mul_dev, with identical
+ * output, but every buffer beyond A, B and C (the evaluation arrays, and the
+ * merge buffer of transforms above the LDS limit) is carved from a fixed
+ * device array of PMLG_NOALLOC_SCRATCH_BYTES per GPU, held by the library
+ * until the process ends; the call makes no hipMalloc, hipFree or other
+ * allocating call.  Limits, each reported as an error before any work is
+ * enqueued: the evaluations must fit the array ((m*k + k*n + m*n) * N words,
+ * plus m*n*N when N exceeds 2^13 (64-bit words) / 2^14 (p < 2^30), N the
+ * transform length 2^ceil(log2(lenA + lenB - 1)): PMLG_ERR_ARG); the product
+ * must be evaluated modulo p itself, not by the multi-prime path
+ * (PMLG_ERR_UNSUPPORTED); and the context's root tables for that transform
+ * length must exist: any earlier product of that length on the context builds
+ * them (PMLG_ERR_UNSUPPORTED otherwise).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (several streams, threads or
+ * contexts) race on it and must be serialised by the caller.  No stage
+ * timing (pmlg_profile_enable) is recorded for these calls. */
+#define PMLG_NOALLOC_SCRATCH_BYTES (256ull << 20)
+pmlg_status pmlg_polmat_mul_dev_noalloc(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA,
+                                        const uint64_t *dB, int64_t m, int64_t k, int64_t n,
+                                        int64_t lenA, int64_t lenB, int method, void *stream);
+
 /* out = (A*B mod x^nhi) div x^nlo, padded length max(nhi-nlo, 0). */
 pmlg_status pmlg_polmat_mulmid(pmlg_ctx ctx, uint64_t *out, const uint64_t *A,
                                const uint64_t *B, int64_t m, int64_t k, int64_t n,

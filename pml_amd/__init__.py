@@ -81,6 +81,10 @@ def lib():
         L.pmlg_polmat_mul.argtypes = [vp, _u64p, _u64p, _u64p, i64, i64, i64, i64, i64, c_int]
         L.pmlg_polmat_mul_dev.restype = st
         L.pmlg_polmat_mul_dev.argtypes = [vp, vp, vp, vp, i64, i64, i64, i64, i64, c_int, vp]
+        if hasattr(L, "pmlg_polmat_mul_dev_noalloc"):
+            L.pmlg_polmat_mul_dev_noalloc.restype = st
+            L.pmlg_polmat_mul_dev_noalloc.argtypes = [vp, vp, vp, vp, i64, i64, i64, i64, i64,
+                                                      c_int, vp]
         L.pmlg_polmat_mulmid.restype = st
         L.pmlg_polmat_mulmid.argtypes = [vp, _u64p, _u64p, _u64p, i64, i64, i64, i64, i64, i64,
                                          i64, c_int]

@@ -202,45 +202,37 @@ const NttTab<uint32_t> &tab_of<uint32_t>(const Tables &T)
 }
 
 // Evaluation/interpolation pipeline for coefficients [lo, hi) of A*B, with a
-// transform length 2^logN that is at least hi and at least lenC - lo.
+// transform length 2^logN that is at least hi and at least lenC - lo, on the
+// given root tables and evaluation buffers (Ah: m*k*N words of W, Bh: k*n*N,
+// Ch: m*n*N, tmp: m*n*N when N exceeds the LDS limit, else unused).  Stage
+// timing (HIP events) only when `prof` is set.
 template <typename W>
-pmlg_status run_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const uint64_t *dA,
-                        const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t strideA,
-                        int64_t strideB, int64_t la, int64_t lb, int64_t lo, int64_t hi, int logN,
-                        PointwiseKind kind, hipStream_t s)
+pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, W *tmp, bool prof,
+                           uint64_t *dOut, int64_t out_stride, const uint64_t *dA,
+                           const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t strideA,
+                           int64_t strideB, int64_t la, int64_t lb, int64_t lo, int64_t hi,
+                           int logN, PointwiseKind kind, hipStream_t s)
 {
-    const Tables *T = nullptr;
-    pmlg_status st = get_tables(ctx, logN, &T);
-    if (st != PMLG_OK)
-        return st;
     const NttTab<W> &tab = tab_of<W>(*T);
     const int64_t N = (int64_t)1 << logN;
     const W p = (W)ctx->p;
-    HIP_TRY(ctx->ev_a.ensure((size_t)(m * k * N) * sizeof(W)), "hipMalloc(eval A)");
-    HIP_TRY(ctx->ev_b.ensure((size_t)(k * n * N) * sizeof(W)), "hipMalloc(eval B)");
-    HIP_TRY(ctx->ev_c.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(eval C)");
-    const int lim = NttLimits<W>::max_log_m;
-    W *tmp = nullptr;
-    if (logN > lim) {
-        HIP_TRY(ctx->tmp.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(tmp)");
-        tmp = static_cast<W *>(ctx->tmp.ptr);
-    }
-    W *Ah = static_cast<W *>(ctx->ev_a.ptr);
-    W *Bh = static_cast<W *>(ctx->ev_b.ptr);
-    W *Ch = static_cast<W *>(ctx->ev_c.ptr);
-
     // point-blocked evaluation layout: 32-byte runs per entry
     const int logP = std::min(sizeof(W) == 8 ? 2 : 3, logN);
     const NttTune &tune = ctx->tune;
     Prof pr;
-    prof_begin(ctx, 0, s, &pr);
+    pr.stage = -1;
+    if (prof)
+        prof_begin(ctx, 0, s, &pr);
     HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, logP, tune, tab, p, s)),
             "ntt_fwd(A)");
-    prof_end(ctx, s, &pr);
-    prof_begin(ctx, 0, s, &pr);
+    if (prof) {
+        prof_end(ctx, s, &pr);
+        prof_begin(ctx, 0, s, &pr);
+    }
     HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, logP, tune, tab, p, s)),
             "ntt_fwd(B)");
-    prof_end(ctx, s, &pr);
+    if (prof)
+        prof_end(ctx, s, &pr);
     // The 8-limb MFMA product leaves V 2^-64 mod p (Montgomery REDC) in [0, 4p);
     // the inverse transform then scales by N^-1 2^64 instead of N^-1.
     ModConst mc = ctx->mc;
@@ -251,16 +243,45 @@ pmlg_status run_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
         itab.ninv = tab.ninvr;
         itab.ninvq = tab.ninvrq;
     }
-    prof_begin(ctx, 1, s, &pr);
+    if (prof)
+        prof_begin(ctx, 1, s, &pr);
     HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, N, logP, mc, s),
             "pointwise");
-    prof_end(ctx, s, &pr);
-    prof_begin(ctx, 2, s, &pr);
+    if (prof) {
+        prof_end(ctx, s, &pr);
+        prof_begin(ctx, 2, s, &pr);
+    }
     HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, logP, tune,
                                          itab, p, tmp, s)),
             "ntt_inv");
-    prof_end(ctx, s, &pr);
+    if (prof)
+        prof_end(ctx, s, &pr);
     return PMLG_OK;
+}
+
+// The same on the context's tables and evaluation buffers (built or grown here).
+template <typename W>
+pmlg_status run_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const uint64_t *dA,
+                        const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t strideA,
+                        int64_t strideB, int64_t la, int64_t lb, int64_t lo, int64_t hi, int logN,
+                        PointwiseKind kind, hipStream_t s)
+{
+    const Tables *T = nullptr;
+    pmlg_status st = get_tables(ctx, logN, &T);
+    if (st != PMLG_OK)
+        return st;
+    const int64_t N = (int64_t)1 << logN;
+    HIP_TRY(ctx->ev_a.ensure((size_t)(m * k * N) * sizeof(W)), "hipMalloc(eval A)");
+    HIP_TRY(ctx->ev_b.ensure((size_t)(k * n * N) * sizeof(W)), "hipMalloc(eval B)");
+    HIP_TRY(ctx->ev_c.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(eval C)");
+    W *tmp = nullptr;
+    if (logN > NttLimits<W>::max_log_m) {
+        HIP_TRY(ctx->tmp.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(tmp)");
+        tmp = static_cast<W *>(ctx->tmp.ptr);
+    }
+    return run_product_on<W>(ctx, T, static_cast<W *>(ctx->ev_a.ptr), static_cast<W *>(ctx->ev_b.ptr),
+                             static_cast<W *>(ctx->ev_c.ptr), tmp, ctx->profile, dOut, out_stride,
+                             dA, dB, m, k, n, strideA, strideB, la, lb, lo, hi, logN, kind, s);
 }
 
 pmlg_status check_ctx(pmlg_ctx ctx)
@@ -351,6 +372,82 @@ pmlg_status mulmid_dev_locked(pmlg_ctx ctx, uint64_t *dOut, const uint64_t *dA,
                               lenB, nlo, nhi, method, s);
 }
 
+}  // namespace pmlg
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of pmlg_polmat_mul_dev_noalloc: the evaluation buffers
+// of one product, in a file-scope device array instead of allocations.  Each
+// device the library runs on holds its own copy for the life of the process.
+// Host code never uses the array's name as an address: its device address is
+// looked up once per device (hipGetSymbolAddress, with that device current)
+// and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint64_t scratch[PMLG_NOALLOC_SCRATCH_BYTES / 8];
+
+namespace pmlg {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this device's scratch array (the device is current)
+pmlg_status scratch_address(int device, void **out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return fail(PMLG_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                            "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return PMLG_OK;
+}
+
+constexpr int64_t align256(int64_t b)
+{
+    return (b + 255) & ~(int64_t)255;
+}
+
+// C = A * B with every buffer beyond A, B and C carved from the scratch
+// array; ctx->mu held, ctx->device current.
+template <typename W>
+pmlg_status mul_noalloc_locked(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA, const uint64_t *dB,
+                               int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
+                               int logN, PointwiseKind kind, hipStream_t s)
+{
+    const int64_t N = (int64_t)1 << logN;
+    const int64_t bA = align256(m * k * N * (int64_t)sizeof(W));
+    const int64_t bB = align256(k * n * N * (int64_t)sizeof(W));
+    const int64_t bC = align256(m * n * N * (int64_t)sizeof(W));
+    const int64_t bT = logN > NttLimits<W>::max_log_m ? bC : 0;
+    const int64_t need = bA + bB + bC + bT;
+    if (need > (int64_t)PMLG_NOALLOC_SCRATCH_BYTES)
+        return fail(PMLG_ERR_ARG, "pmlg_polmat_mul_dev_noalloc: this product needs " +
+                                      std::to_string(need) + " bytes of evaluation scratch, more "
+                                      "than the fixed " +
+                                      std::to_string((int64_t)PMLG_NOALLOC_SCRATCH_BYTES) +
+                                      " bytes; use pmlg_polmat_mul_dev");
+    void *base = nullptr;
+    pmlg_status st = scratch_address(ctx->device, &base);
+    if (st != PMLG_OK)
+        return st;
+    unsigned char *b = static_cast<unsigned char *>(base);
+    W *Ah = reinterpret_cast<W *>(b);
+    W *Bh = reinterpret_cast<W *>(b + bA);
+    W *Ch = reinterpret_cast<W *>(b + bA + bB);
+    W *tmp = bT ? reinterpret_cast<W *>(b + bA + bB + bC) : nullptr;
+    const int64_t lenC = lenA + lenB - 1;
+    return run_product_on<W>(ctx, &ctx->tabs[logN], Ah, Bh, Ch, tmp, false, dC, lenC, dA, dB, m,
+                             k, n, lenA, lenB, lenA, lenB, 0, lenC, logN, kind, s);
+}
+
+}  // namespace
 }  // namespace pmlg
 
 // ---------------------------------------------------------------------------
@@ -493,6 +590,49 @@ pmlg_status pmlg_polmat_mul_dev(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA,
         return lenA < 0 || lenB < 0 ? fail(PMLG_ERR_ARG, "negative length") : PMLG_OK;
     return pmlg_polmat_mulmid_dev(ctx, dC, dA, dB, m, k, n, lenA, lenB, 0, lenA + lenB - 1,
                                   method, stream);
+}
+
+pmlg_status pmlg_polmat_mul_dev_noalloc(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA,
+                                        const uint64_t *dB, int64_t m, int64_t k, int64_t n,
+                                        int64_t lenA, int64_t lenB, int method, void *stream)
+{
+    pmlg_status st = check_ctx(ctx);
+    if (st != PMLG_OK)
+        return st;
+    if (m < 0 || k < 0 || n < 0 || lenA < 0 || lenB < 0)
+        return fail(PMLG_ERR_ARG, "negative dimension or length");
+    if (m > INT32_MAX / 64 || k > INT32_MAX / 64 || n > INT32_MAX / 64)
+        return fail(PMLG_ERR_ARG, "dimension too large");
+    if (lenA == 0 || lenB == 0 || m == 0 || n == 0)
+        return PMLG_OK;
+    if (!dC || (k > 0 && (!dA || !dB)))
+        return fail(PMLG_ERR_ARG, "null buffer");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const int64_t lenC = lenA + lenB - 1;
+    if (k == 0) {
+        HIP_TRY(fill_zero(dC, m * n * lenC, s), "fill_zero");
+        return PMLG_OK;
+    }
+    const int logN = ceil_log2(lenC);
+    if (logN > ctx->max_log)
+        return fail(PMLG_ERR_UNSUPPORTED,
+                    "pmlg_polmat_mul_dev_noalloc: transform length 2^" + std::to_string(logN) +
+                        " needs the multi-prime path, which allocates; use pmlg_polmat_mul_dev");
+    if ((int)ctx->tabs.size() <= logN || !ctx->tabs[logN].mem)
+        return fail(PMLG_ERR_UNSUPPORTED,
+                    "pmlg_polmat_mul_dev_noalloc: the root tables of transform length 2^" +
+                        std::to_string(logN) +
+                        " are not built on this context; any earlier product of that transform "
+                        "length on it (e.g. one pmlg_polmat_mul_dev call) builds them");
+    PointwiseKind kind;
+    st = pick_kind(ctx, method, m, k, n, &kind);
+    if (st != PMLG_OK)
+        return st;
+    if (ctx->small)
+        return mul_noalloc_locked<uint32_t>(ctx, dC, dA, dB, m, k, n, lenA, lenB, logN, kind, s);
+    return mul_noalloc_locked<uint64_t>(ctx, dC, dA, dB, m, k, n, lenA, lenB, logN, kind, s);
 }
 
 pmlg_status pmlg_polmat_mulmid(pmlg_ctx ctx, uint64_t *out, const uint64_t *A, const uint64_t *B,
