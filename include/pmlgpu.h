@@ -26,11 +26,12 @@
  *   pmlg_polmat_mul_batch: no single reference counterpart; it runs many
  *                          independent nmod_poly_mat_multiply calls at once.
  *
- * Supported moduli: any p in [2, 2^62).  When 2^ceil(log2(N)) | p-1 for the
- * transform length N a product needs (N <= 2^15 for p >= 2^30, 2^16 for
- * p < 2^30), the product is evaluated modulo p itself; otherwise (non-FFT
- * primes, composite moduli) it is computed modulo three 61-bit FFT primes and
- * recombined by CRT (N <= 2^15).  Approximant bases need an odd prime.
+ * Supported moduli: any p in [2, 2^64).  When p < 2^62 is prime and
+ * 2^ceil(log2(N)) | p-1 for the transform length N a product needs (N <= 2^24
+ * for p >= 2^30, 2^26 for p < 2^30), the product is evaluated modulo p
+ * itself; otherwise (non-FFT primes, composite moduli, p >= 2^62) it is
+ * computed modulo three 61-bit FFT primes and recombined by CRT (N <= 2^24,
+ * k min(lenA, lenB) < 2^52).  Approximant and kernel bases need an odd prime.
  * Longer products return PMLG_ERR_UNSUPPORTED.
  */
 #ifndef PMLGPU_H
@@ -144,8 +145,11 @@ pmlg_status pmlg_polmat_mulmid_entries(pmlg_ctx ctx, uint64_t *const *out, int64
  * coefficient d at P[(i*m + j)*capP + d]; every coefficient up to capP is
  * written, zeros past the basis); *lenP its normalised length;
  * shift (length m) is the input shift on entry and the shifted row degree of
- * P on return.  1 <= m, n <= 128.  The products inside PM-Basis run on the
- * device with the given pointwise method.  pmlg_mbasis returns
+ * P on return.  1 <= m <= 256, 1 <= n <= 128, with n (m | 1) words of the
+ * transposed residual and the step's buffers within one workgroup's LDS
+ * (PMLG_ERR_UNSUPPORTED otherwise; 160 x 80 fits, 256 x 62 with 64-bit words,
+ * 256 x 128 for p < 2^30).  The products inside PM-Basis run on the device
+ * with the given pointwise method.  pmlg_mbasis returns
  * PMLG_ERR_UNSUPPORTED when m^2 (order + 1) + m n order >= 2^31 (PM-Basis
  * has no such limit: its M-Basis leaves have order at most 256). */
 pmlg_status pmlg_pmbasis(pmlg_ctx ctx, uint64_t *P, int64_t *lenP, int64_t *shift,
@@ -188,7 +192,7 @@ pmlg_status pmlg_pmbasis_leaf_order(pmlg_ctx ctx, int64_t m, int64_t n, int meth
  * basis rows (shift holds the input shift on entry).  Every approximant
  * basis, product and middle product runs on the device.  When capK < *lenK,
  * K is not written and PMLG_ERR_ARG is returned with *lenK and *nullity set:
- * call again with capK >= *lenK.  1 <= m <= 128; n <= 128 for
+ * call again with capK >= *lenK.  1 <= m <= 256; n <= 128 for
  * PMLG_KERNEL_VIA_APPROX; the modulus must be an odd prime. */
 #define PMLG_KERNEL_ZLS 0
 #define PMLG_KERNEL_VIA_APPROX 1
@@ -247,6 +251,17 @@ pmlg_status pmlg_batch_plan_sync(pmlg_batch_plan plan);
  * ms[0] forward transforms, ms[1] direct convolutions and pointwise
  * products, ms[2] inverse transforms (averages over `reps` runs). */
 pmlg_status pmlg_batch_plan_profile(pmlg_batch_plan plan, int reps, double *ms);
+/* Stage timing of graph replays: the plan's next run captures one graph per
+ * stage (forward transforms | direct convolutions and pointwise products |
+ * inverse transforms); later runs launch the three graphs in turn on the
+ * stream with HIP events recorded before, between and after them, from
+ * `ring` event sets used in turn.  ring 0 turns it off.  The plan needs
+ * PMLG_BATCH_GRAPH. */
+pmlg_status pmlg_batch_plan_time_stages(pmlg_batch_plan plan, int ring);
+/* Waits for the timed replays and returns the stage times (ms[0..2])
+ * averaged over each event set's last replay since the previous call, and
+ * the number of replays averaged; the counts restart at zero. */
+pmlg_status pmlg_batch_plan_stage_times(pmlg_batch_plan plan, double *ms, int64_t *replays);
 /* Configuration chosen for product i (-1 for an empty product). */
 pmlg_status pmlg_batch_plan_config(pmlg_batch_plan plan, int64_t i, int *config);
 /* Kernel launches per run and arena bytes. */

@@ -114,6 +114,10 @@ def lib():
         L.pmlg_batch_plan_sync.argtypes = [vp]
         L.pmlg_batch_plan_profile.restype = st
         L.pmlg_batch_plan_profile.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_double)]
+        L.pmlg_batch_plan_time_stages.restype = st
+        L.pmlg_batch_plan_time_stages.argtypes = [vp, c_int]
+        L.pmlg_batch_plan_stage_times.restype = st
+        L.pmlg_batch_plan_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), _i64p]
         L.pmlg_batch_plan_config.restype = st
         L.pmlg_batch_plan_config.argtypes = [vp, i64, ctypes.POINTER(c_int)]
         L.pmlg_batch_plan_stats.restype = st
@@ -453,6 +457,19 @@ class BatchPlan:
         ms = (ctypes.c_double * 3)()
         _check(lib().pmlg_batch_plan_profile(self.handle, reps, ms))
         return list(ms)
+
+    def time_stages(self, ring):
+        """Capture `ring` graph instances with events between the stages; the
+        following replays use them in turn (stage_times reads them)."""
+        _check(lib().pmlg_batch_plan_time_stages(self.handle, int(ring)))
+
+    def stage_times(self):
+        """(ms per stage averaged over the timed replays since the last call,
+        number of replays averaged)."""
+        ms = (ctypes.c_double * 3)()
+        n = ctypes.c_int64()
+        _check(lib().pmlg_batch_plan_stage_times(self.handle, ms, ctypes.byref(n)))
+        return list(ms), n.value
 
     def config(self, i):
         c = ctypes.c_int()

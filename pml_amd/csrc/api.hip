@@ -4,6 +4,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ctx.hpp"
@@ -43,7 +44,12 @@ int ceil_log2(int64_t x)
     return l;
 }
 
-// Builds (once) the root tables for transform length 2^logN.
+// Builds (once) the root tables for transform length 2^logN.  Up to 4 times
+// the LDS limit: the full tables w^t, w^-t (t < N, with Shoup companions)
+// then the per-stage tables of the LDS sub-transform.  Longer: the
+// sub-transform's per-stage tables, then the column pass's tables (w_R^k and
+// w_R^-k for k < R/2, and the two-level twist tables w^a, w^-a = lo[a mod S]
+// * hi[a / S], S = 2^ceil(logN / 2)), so tables stay O(sqrt(N) + R + M).
 pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
 {
     if ((int)ctx->tabs.size() <= logN)
@@ -52,32 +58,45 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
     if (!T.mem) {
         const int64_t N = (int64_t)1 << logN;
         const uint64_t p = ctx->p;
-        uint64_t w = host_powmod(ctx->root, (uint64_t)1 << (ctx->max_log - logN), p);
-        uint64_t wi = host_invmod(w, p);
+        const uint64_t w = host_powmod(ctx->root, (uint64_t)1 << (ctx->max_log - logN), p);
+        const uint64_t wi = host_invmod(w, p);
         const size_t wsz = ctx->small ? 4 : 8;
-        // full tables (4N words) followed by the per-stage tables of the
-        // LDS sub-transform (2M words forward, 2M words inverse)
         const int lim = ctx->small ? NttLimits<uint32_t>::max_log_m : NttLimits<uint64_t>::max_log_m;
         const int logM = logN < lim ? logN : lim;
         const int64_t M = (int64_t)1 << logM;
-        std::vector<unsigned char> h((4 * N + 4 * M) * wsz);
-        uint64_t x = 1 % p, y = 1 % p;
-        for (int64_t t = 0; t < N; ++t) {
-            if (ctx->small) {
-                uint32_t *a = reinterpret_cast<uint32_t *>(h.data());
-                a[t] = (uint32_t)x;
-                a[N + t] = (uint32_t)host_shoup_pre32(x, p);
-                a[2 * N + t] = (uint32_t)y;
-                a[3 * N + t] = (uint32_t)host_shoup_pre32(y, p);
-            } else {
-                uint64_t *a = reinterpret_cast<uint64_t *>(h.data());
-                a[t] = x;
-                a[N + t] = host_shoup_pre64(x, p);
-                a[2 * N + t] = y;
-                a[3 * N + t] = host_shoup_pre64(y, p);
+        const bool longN = logN > lim + 2;
+        const int logR = logN - logM;
+        const int64_t R = (int64_t)1 << logR;
+        const int slog = (logN + 1) / 2;
+        const int64_t S = (int64_t)1 << slog, H = N >> slog;
+        // word offsets: [full 4N | stage 4M] or [stage 4M | rt 4 (R/2) | lo, hi (4 (S + H))]
+        const int64_t full = longN ? 0 : 4 * N;
+        const int64_t stage0 = full;
+        const int64_t rt0 = stage0 + 4 * M;
+        const int64_t tw0 = rt0 + (longN ? 2 * R : 0);
+        const int64_t words = tw0 + (longN ? 4 * (S + H) : 0);
+        std::vector<unsigned char> h((size_t)words * wsz);
+        auto put = [&](int64_t at, uint64_t v) {
+            if (ctx->small)
+                reinterpret_cast<uint32_t *>(h.data())[at] = (uint32_t)v;
+            else
+                reinterpret_cast<uint64_t *>(h.data())[at] = v;
+        };
+        auto pre = [&](uint64_t v) {
+            return ctx->small ? host_shoup_pre32(v, p) : host_shoup_pre64(v, p);
+        };
+        // value and companion of x^j, j < cnt, at [at, at + cnt) and [at + cnt, at + 2 cnt)
+        auto powers = [&](int64_t at, int64_t cnt, uint64_t x) {
+            uint64_t v = 1 % p;
+            for (int64_t j = 0; j < cnt; ++j) {
+                put(at + j, v);
+                put(at + cnt + j, pre(v));
+                v = host_mulmod(v, x, p);
             }
-            x = host_mulmod(x, w, p);
-            y = host_mulmod(y, wi, p);
+        };
+        if (!longN) {
+            powers(0, N, w);       // tw, twq
+            powers(2 * N, N, wi);  // itw, itwq
         }
         {
             // sub-transform root w_M = w^(N/M); stage s entry x = w_M^(x 2^s)
@@ -90,25 +109,25 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
                 const uint64_t stepi = host_powmod(wmi, (uint64_t)1 << s, p);
                 uint64_t a = 1 % p, b = 1 % p;
                 for (int64_t x = 0; x < cnt; ++x) {
-                    const int64_t f = 4 * N + 2 * (off + x);  // forward pair index (words)
-                    const int64_t g = 4 * N + 2 * M + 2 * (off + x);
-                    if (ctx->small) {
-                        uint32_t *d = reinterpret_cast<uint32_t *>(h.data());
-                        d[f] = (uint32_t)a;
-                        d[f + 1] = (uint32_t)host_shoup_pre32(a, p);
-                        d[g] = (uint32_t)b;
-                        d[g + 1] = (uint32_t)host_shoup_pre32(b, p);
-                    } else {
-                        uint64_t *d = reinterpret_cast<uint64_t *>(h.data());
-                        d[f] = a;
-                        d[f + 1] = host_shoup_pre64(a, p);
-                        d[g] = b;
-                        d[g + 1] = host_shoup_pre64(b, p);
-                    }
+                    const int64_t f = stage0 + 2 * (off + x);  // forward pair index (words)
+                    const int64_t g = stage0 + 2 * M + 2 * (off + x);
+                    put(f, a);
+                    put(f + 1, pre(a));
+                    put(g, b);
+                    put(g + 1, pre(b));
                     a = host_mulmod(a, step, p);
                     b = host_mulmod(b, stepi, p);
                 }
             }
+        }
+        if (longN) {
+            const uint64_t wr = host_powmod(w, (uint64_t)M, p), wri = host_invmod(wr, p);
+            powers(rt0, R / 2, wr);          // rt, rtq
+            powers(rt0 + R, R / 2, wri);     // rti, rtiq
+            powers(tw0, S, w);               // lo, loq
+            powers(tw0 + 2 * S, H, host_powmod(w, (uint64_t)S, p));     // hi, hiq
+            powers(tw0 + 2 * S + 2 * H, S, wi);                         // loi, loiq
+            powers(tw0 + 4 * S + 2 * H, H, host_powmod(wi, (uint64_t)S, p));  // hii, hiiq
         }
         void *mem = nullptr;
         HIP_TRY(hipMalloc(&mem, h.size()), "hipMalloc(twiddles)");
@@ -120,18 +139,39 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
         T.mem = mem;
         const uint64_t ninv = host_invmod((uint64_t)(N % p), p);
         const uint64_t ninvr = host_mulmod(ninv, ctx->mc.r64, p);
-        if (ctx->small) {
-            const uint32_t *a = reinterpret_cast<const uint32_t *>(mem);
-            T.t32 = NttTab<uint32_t>{a, a + N, a + 2 * N, a + 3 * N, (uint32_t)ninv,
-                                     (uint32_t)host_shoup_pre32(ninv, p), a + 4 * N,
-                                     a + 4 * N + 2 * M, (uint32_t)ninvr,
-                                     (uint32_t)host_shoup_pre32(ninvr, p)};
-        } else {
-            const uint64_t *a = reinterpret_cast<const uint64_t *>(mem);
-            T.t64 = NttTab<uint64_t>{a, a + N, a + 2 * N, a + 3 * N, ninv,
-                                     host_shoup_pre64(ninv, p), a + 4 * N, a + 4 * N + 2 * M,
-                                     ninvr, host_shoup_pre64(ninvr, p)};
-        }
+        auto fill = [&](auto *a, auto &tab, auto &col) {
+            using Wt = std::remove_const_t<std::remove_pointer_t<decltype(a)>>;
+            const Wt *full_tw = longN ? nullptr : a;
+            tab.tw = full_tw;
+            tab.twq = longN ? nullptr : a + N;
+            tab.itw = longN ? nullptr : a + 2 * N;
+            tab.itwq = longN ? nullptr : a + 3 * N;
+            tab.ninv = (Wt)ninv;
+            tab.ninvq = (Wt)pre(ninv);
+            tab.stw = a + stage0;
+            tab.sitw = a + stage0 + 2 * M;
+            tab.ninvr = (Wt)ninvr;
+            tab.ninvrq = (Wt)pre(ninvr);
+            if (longN) {
+                col.rt = a + rt0;
+                col.rtq = a + rt0 + R / 2;
+                col.rti = a + rt0 + R;
+                col.rtiq = a + rt0 + R + R / 2;
+                col.lo = a + tw0;
+                col.loq = a + tw0 + S;
+                col.hi = a + tw0 + 2 * S;
+                col.hiq = a + tw0 + 2 * S + H;
+                col.loi = a + tw0 + 2 * S + 2 * H;
+                col.loiq = a + tw0 + 3 * S + 2 * H;
+                col.hii = a + tw0 + 4 * S + 2 * H;
+                col.hiiq = a + tw0 + 4 * S + 3 * H;
+                col.slog = slog;
+            }
+        };
+        if (ctx->small)
+            fill(reinterpret_cast<const uint32_t *>(mem), T.t32, T.c32);
+        else
+            fill(reinterpret_cast<const uint64_t *>(mem), T.t64, T.c64);
     }
     *out = &T;
     return PMLG_OK;
@@ -201,19 +241,52 @@ const NttTab<uint32_t> &tab_of<uint32_t>(const Tables &T)
     return T.t32;
 }
 
+template <typename W>
+const ColTab<W> &col_of(const Tables &T);
+template <>
+const ColTab<uint64_t> &col_of<uint64_t>(const Tables &T)
+{
+    return T.c64;
+}
+template <>
+const ColTab<uint32_t> &col_of<uint32_t>(const Tables &T)
+{
+    return T.c32;
+}
+
+// Bytes of the transforms' work buffer for an m x k by k x n product at
+// length 2^logN (0 when N fits 4 times the LDS limit... below it): the
+// inverse's m*n*N words of W above the LDS limit, and the forward column
+// pass's max(m*k, k*n)*N 64-bit words for long transforms.
+template <typename W>
+int64_t product_tmp_bytes(int64_t m, int64_t k, int64_t n, int logN)
+{
+    const int lim = NttLimits<W>::max_log_m;
+    if (logN <= lim)
+        return 0;
+    const int64_t N = (int64_t)1 << logN;
+    int64_t b = m * n * N * (int64_t)sizeof(W);
+    if (logN > lim + 2)
+        b = std::max<int64_t>(b, std::max(m * k, k * n) * N * 8);
+    return b;
+}
+
 // Evaluation/interpolation pipeline for coefficients [lo, hi) of A*B, with a
 // transform length 2^logN that is at least hi and at least lenC - lo, on the
 // given root tables and evaluation buffers (Ah: m*k*N words of W, Bh: k*n*N,
 // Ch: m*n*N, tmp: m*n*N when N exceeds the LDS limit, else unused).  Stage
 // timing (HIP events) only when `prof` is set.
 template <typename W>
-pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, W *tmp, bool prof,
+pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, void *tmpv, bool prof,
                            uint64_t *dOut, int64_t out_stride, const uint64_t *dA,
                            const uint64_t *dB, int64_t m, int64_t k, int64_t n, int64_t strideA,
                            int64_t strideB, int64_t la, int64_t lb, int64_t lo, int64_t hi,
                            int logN, PointwiseKind kind, hipStream_t s)
 {
     const NttTab<W> &tab = tab_of<W>(*T);
+    const ColTab<W> *ct = &col_of<W>(*T);
+    W *tmp = static_cast<W *>(tmpv);                   // inverse work buffer
+    uint64_t *tmpf = static_cast<uint64_t *>(tmpv);    // forward column pass
     const int64_t N = (int64_t)1 << logN;
     const W p = (W)ctx->p;
     // point-blocked evaluation layout: 32-byte runs per entry
@@ -223,13 +296,15 @@ pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, W
     pr.stage = -1;
     if (prof)
         prof_begin(ctx, 0, s, &pr);
-    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, logP, tune, tab, p, s)),
+    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, logP, tune, tab, p, s,
+                                         tmpf, ct)),
             "ntt_fwd(A)");
     if (prof) {
         prof_end(ctx, s, &pr);
         prof_begin(ctx, 0, s, &pr);
     }
-    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, logP, tune, tab, p, s)),
+    HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, logP, tune, tab, p, s,
+                                         tmpf, ct)),
             "ntt_fwd(B)");
     if (prof)
         prof_end(ctx, s, &pr);
@@ -252,7 +327,7 @@ pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, W
         prof_begin(ctx, 2, s, &pr);
     }
     HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, logP, tune,
-                                         itab, p, tmp, s)),
+                                         itab, p, tmp, s, ct)),
             "ntt_inv");
     if (prof)
         prof_end(ctx, s, &pr);
@@ -274,10 +349,10 @@ pmlg_status run_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
     HIP_TRY(ctx->ev_a.ensure((size_t)(m * k * N) * sizeof(W)), "hipMalloc(eval A)");
     HIP_TRY(ctx->ev_b.ensure((size_t)(k * n * N) * sizeof(W)), "hipMalloc(eval B)");
     HIP_TRY(ctx->ev_c.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(eval C)");
-    W *tmp = nullptr;
-    if (logN > NttLimits<W>::max_log_m) {
-        HIP_TRY(ctx->tmp.ensure((size_t)(m * n * N) * sizeof(W)), "hipMalloc(tmp)");
-        tmp = static_cast<W *>(ctx->tmp.ptr);
+    void *tmp = nullptr;
+    if (const int64_t tb = product_tmp_bytes<W>(m, k, n, logN)) {
+        HIP_TRY(ctx->tmp.ensure((size_t)tb), "hipMalloc(tmp)");
+        tmp = ctx->tmp.ptr;
     }
     return run_product_on<W>(ctx, T, static_cast<W *>(ctx->ev_a.ptr), static_cast<W *>(ctx->ev_b.ptr),
                              static_cast<W *>(ctx->ev_c.ptr), tmp, ctx->profile, dOut, out_stride,
@@ -425,7 +500,7 @@ pmlg_status mul_noalloc_locked(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA, c
     const int64_t bA = align256(m * k * N * (int64_t)sizeof(W));
     const int64_t bB = align256(k * n * N * (int64_t)sizeof(W));
     const int64_t bC = align256(m * n * N * (int64_t)sizeof(W));
-    const int64_t bT = logN > NttLimits<W>::max_log_m ? bC : 0;
+    const int64_t bT = align256(product_tmp_bytes<W>(m, k, n, logN));
     const int64_t need = bA + bB + bC + bT;
     if (need > (int64_t)PMLG_NOALLOC_SCRATCH_BYTES)
         return fail(PMLG_ERR_ARG, "pmlg_polmat_mul_dev_noalloc: this product needs " +
@@ -441,7 +516,7 @@ pmlg_status mul_noalloc_locked(pmlg_ctx ctx, uint64_t *dC, const uint64_t *dA, c
     W *Ah = reinterpret_cast<W *>(b);
     W *Bh = reinterpret_cast<W *>(b + bA);
     W *Ch = reinterpret_cast<W *>(b + bA + bB);
-    W *tmp = bT ? reinterpret_cast<W *>(b + bA + bB + bC) : nullptr;
+    void *tmp = bT ? static_cast<void *>(b + bA + bB + bC) : nullptr;
     const int64_t lenC = lenA + lenB - 1;
     return run_product_on<W>(ctx, &ctx->tabs[logN], Ah, Bh, Ch, tmp, false, dC, lenC, dA, dB, m,
                              k, n, lenA, lenB, lenA, lenB, 0, lenC, logN, kind, s);
@@ -465,8 +540,8 @@ pmlg_status pmlg_ctx_create(pmlg_ctx *out, uint64_t p, int device)
     if (!out)
         return fail(PMLG_ERR_ARG, "null output pointer");
     *out = nullptr;
-    if (p < 2 || p >= ((uint64_t)1 << 62))
-        return fail(PMLG_ERR_UNSUPPORTED, "modulus must be in [2, 2^62)");
+    if (p < 2)
+        return fail(PMLG_ERR_UNSUPPORTED, "modulus must be at least 2");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(PMLG_ERR_DEVICE, "no HIP device available");
@@ -479,14 +554,18 @@ pmlg_status pmlg_ctx_create(pmlg_ctx *out, uint64_t p, int device)
     ctx->device = device;
     ctx->small = p < ((uint64_t)1 << 30);
     ctx->is_prime = p > 2 && host_is_prime(p);
-    if (ctx->is_prime) {
+    // the transforms keep values below 4p in 64-bit words: p >= 2^62 takes
+    // the multi-prime path for every product
+    const bool transforms = ctx->is_prime && p < ((uint64_t)1 << 62);
+    if (transforms) {
         int v2 = __builtin_ctzll(p - 1);
-        const int cap = ctx->small ? NttLimits<uint32_t>::max_log_m + 2
-                                   : NttLimits<uint64_t>::max_log_m + 2;
+        const int cap = ctx->small ? NttLimits<uint32_t>::max_log_n
+                                   : NttLimits<uint64_t>::max_log_n;
         ctx->max_log = v2 < cap ? v2 : cap;
         ctx->root = host_root_of_unity_2k(p, ctx->max_log);
     } else {
-        // no transform modulo p itself: every product takes the multi-prime path
+        // no transform modulo p itself (composite, or p >= 2^62): every product
+        // takes the multi-prime path
         ctx->max_log = -1;
         ctx->root = 0;
     }

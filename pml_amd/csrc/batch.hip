@@ -16,6 +16,7 @@
 // threshold rule, in that order; the result is the same exact product
 // whichever runs.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
 #include <atomic>
@@ -155,7 +156,11 @@ bool config_valid(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, i
     const ProdConfig &cf = kConfigs[c];
     if (cf.school)
         return std::min(lenA, lenB) <= kSchoolMaxLen;
-    if (lenA > 0 && lenB > 0 && ceil_log2(lenA + lenB - 1) > ctx->max_log)
+    // the segment-table transforms go up to 4 times the LDS limit
+    const int seg_max = ctx->small ? NttLimits<uint32_t>::max_log_m + 2
+                                   : NttLimits<uint64_t>::max_log_m + 2;
+    if (lenA > 0 && lenB > 0 &&
+        ceil_log2(lenA + lenB - 1) > std::min(ctx->max_log, seg_max))
         return false;
     if (cf.kind == PW_MFMA)
         return ctx->mc.limbs > 0 && mfma_supported((int)m, (int)k, (int)n, ctx->mc.limbs, ctx->p);
@@ -163,11 +168,49 @@ bool config_valid(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, i
 }
 
 // Threshold rule (the fallback, and the baseline the learned choice is
-// measured against):
-// direct convolution when both operands are short, else transforms with the
-// MFMA pointwise kernel whenever the shape and modulus allow it.
+// measured against), banded in the style of the reference's dispatch
+// (mul.c:57-96: bands over the dimension and the length): the word size, the
+// shorter operand's length (<= 8, 32, 128, 512, 2048, longer), the largest
+// dimension (<= 8, 16, 32, 64) and the product length (< 2^11, 2^13, 2^15,
+// longer) select a cell, whose configuration was fitted on the recorded
+// timings (bands.inc, generated by `python -m pml_amd.learn bands`; the same
+// cell function is pml_amd.learn.band_index).  A cell without a fit, or whose
+// configuration cannot run the shape, falls back to the fixed rule: direct
+// convolution when both operands are short, else transforms with the MFMA
+// pointwise kernel whenever the shape and modulus allow it, else plain.
+#include "bands.inc"
+
+int band_index(bool small, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB)
+{
+    static const int64_t kShort[5] = {8, 32, 128, 512, 2048};
+    static const int64_t kDim[3] = {8, 16, 32};
+    const int w = small ? 0 : 1;
+    const int64_t sh = std::min(lenA, lenB);
+    int sb = 5;
+    for (int i = 0; i < 5; ++i)
+        if (sh <= kShort[i]) {
+            sb = i;
+            break;
+        }
+    const int64_t dim = std::max(m, std::max(k, n));
+    int db = 3;
+    for (int i = 0; i < 3; ++i)
+        if (dim <= kDim[i]) {
+            db = i;
+            break;
+        }
+    const int lg = ceil_log2(lenA + lenB - 1);
+    const int lb = std::min(3, std::max(0, (lg - 9) / 2));
+    return ((w * 6 + sb) * 4 + db) * 4 + lb;
+}
+
 int threshold_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB)
 {
+    if (lenA > 0 && lenB > 0) {
+        const int c = kBandConfig[band_index(ctx->small, m, k, n, lenA, lenB)];
+        if (c >= 0 && config_valid(ctx, m, k, n, lenA, lenB, c))
+            return c;
+    }
     if (std::min(lenA, lenB) <= kSchoolThreshold && config_valid(ctx, m, k, n, lenA, lenB, 0))
         return 0;
     if (config_valid(ctx, m, k, n, lenA, lenB, 2))
@@ -297,6 +340,16 @@ struct pmlg_batch_plan_s {
     std::vector<ProdInfo> prods;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // stage timing of graph replays (pmlg_batch_plan_time_stages): one graph
+    // per stage (rgraph, rexec), launched in turn with events recorded
+    // between them; `ring` sets of events used in turn, replays[r] counting
+    // the replays of set r since the last read
+    int ring = 0;
+    std::vector<hipGraph_t> rgraph;
+    std::vector<hipGraphExec_t> rexec;
+    std::vector<std::array<hipEvent_t, 4>> rev;
+    std::vector<int64_t> replays;
+    int64_t rnext = 0;
 };
 
 namespace {
@@ -391,16 +444,23 @@ struct StageEvents {
     bool on;
 };
 
-pmlg_status launch_all(pmlg_batch_plan plan, hipStream_t s, StageEvents *se = nullptr)
+// `only` in 0..2 launches that stage alone (the per-stage graphs of
+// pmlg_batch_plan_time_stages), -1 all three
+pmlg_status launch_all(pmlg_batch_plan plan, hipStream_t s, StageEvents *se = nullptr,
+                       int only = -1)
 {
     const unsigned char *tb = static_cast<const unsigned char *>(plan->tables);
     pmlg_status st;
     if (se)
         HIP_TRY(hipEventRecord(se->ev[0], s), "hipEventRecord");
-    if ((st = launch_groups<uint64_t>(plan, s)) != PMLG_OK || (st = launch_groups<uint32_t>(plan, s)) != PMLG_OK)
-        return st;
+    if (only < 0 || only == 0)
+        if ((st = launch_groups<uint64_t>(plan, s)) != PMLG_OK ||
+            (st = launch_groups<uint32_t>(plan, s)) != PMLG_OK)
+            return st;
     if (se)
         HIP_TRY(hipEventRecord(se->ev[1], s), "hipEventRecord");
+    if (only >= 0 && only != 1)
+        goto inverse;
     for (const SchoolGroup &g : plan->school) {
         const int grid = (int)std::min<int64_t>(g.blocks, 2048);
         hipLaunchKernelGGL(school_segs, dim3(grid), dim3(kSchoolThreads), 0, s,
@@ -415,10 +475,13 @@ pmlg_status launch_all(pmlg_batch_plan plan, hipStream_t s, StageEvents *se = nu
     }
     if ((st = launch_pw<uint64_t>(plan, s)) != PMLG_OK || (st = launch_pw<uint32_t>(plan, s)) != PMLG_OK)
         return st;
+inverse:
     if (se)
         HIP_TRY(hipEventRecord(se->ev[2], s), "hipEventRecord");
-    if ((st = launch_inv<uint64_t>(plan, s)) != PMLG_OK || (st = launch_inv<uint32_t>(plan, s)) != PMLG_OK)
-        return st;
+    if (only < 0 || only == 2)
+        if ((st = launch_inv<uint64_t>(plan, s)) != PMLG_OK ||
+            (st = launch_inv<uint32_t>(plan, s)) != PMLG_OK)
+            return st;
     if (se)
         HIP_TRY(hipEventRecord(se->ev[3], s), "hipEventRecord");
     return PMLG_OK;
@@ -520,6 +583,25 @@ void finish_groups(pmlg_batch_plan plan, Blob &blob,
     }
 }
 
+void free_ring(pmlg_batch_plan plan)
+{
+    for (hipGraphExec_t e : plan->rexec)
+        if (e)
+            (void)hipGraphExecDestroy(e);
+    for (hipGraph_t g : plan->rgraph)
+        if (g)
+            (void)hipGraphDestroy(g);
+    for (auto &ev : plan->rev)
+        for (hipEvent_t e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+    plan->rexec.clear();
+    plan->rgraph.clear();
+    plan->rev.clear();
+    plan->replays.clear();
+    plan->rnext = 0;
+}
+
 void free_plan_memory(pmlg_batch_plan plan)
 {
     DeviceGuard g(plan->device);
@@ -531,6 +613,7 @@ void free_plan_memory(pmlg_batch_plan plan)
     plan->graph = nullptr;
     if (plan->stream)
         (void)hipStreamSynchronize(plan->stream);
+    free_ring(plan);
     if (plan->own_mem) {
         if (plan->arena)
             (void)hipFree(plan->arena);
@@ -717,7 +800,46 @@ pmlg_status run_plan(pmlg_batch_plan plan, hipStream_t s)
         HIP_TRY(e, "hipStreamEndCapture");
         plan->graph = graph;
         HIP_TRY(hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        // one graph per stage, for timing the replays with events between them
+        if (plan->ring > 0) {
+            for (int stg = 0; stg < 3; ++stg) {
+                HIP_TRY(hipStreamBeginCapture(plan->stream, hipStreamCaptureModeThreadLocal),
+                        "hipStreamBeginCapture");
+                pmlg_status st2 = launch_all(plan, plan->stream, nullptr, stg);
+                hipGraph_t g2 = nullptr;
+                hipError_t e2 = hipStreamEndCapture(plan->stream, &g2);
+                if (st2 != PMLG_OK) {
+                    if (g2)
+                        (void)hipGraphDestroy(g2);
+                    return st2;
+                }
+                HIP_TRY(e2, "hipStreamEndCapture");
+                plan->rgraph.push_back(g2);
+                hipGraphExec_t x2 = nullptr;
+                HIP_TRY(hipGraphInstantiate(&x2, g2, nullptr, nullptr, 0), "hipGraphInstantiate");
+                plan->rexec.push_back(x2);
+            }
+            for (int r = 0; r < plan->ring; ++r) {
+                std::array<hipEvent_t, 4> ev{};
+                for (hipEvent_t &x : ev)
+                    HIP_TRY(hipEventCreate(&x), "hipEventCreate");
+                plan->rev.push_back(ev);
+                plan->replays.push_back(0);
+            }
+        }
         return PMLG_OK;  // this run's results come from the eager pass
+    }
+    if (!plan->rexec.empty()) {
+        // the three stage graphs with events recorded between them
+        const size_t r = (size_t)(plan->rnext++ % (int64_t)plan->rev.size());
+        const std::array<hipEvent_t, 4> &ev = plan->rev[r];
+        for (int stg = 0; stg < 3; ++stg) {
+            HIP_TRY(hipEventRecord(ev[stg], s), "hipEventRecord");
+            HIP_TRY(hipGraphLaunch(plan->rexec[stg], s), "hipGraphLaunch");
+        }
+        HIP_TRY(hipEventRecord(ev[3], s), "hipEventRecord");
+        plan->replays[r]++;
+        return PMLG_OK;
     }
     HIP_TRY(hipGraphLaunch(plan->exec, s), "hipGraphLaunch");
     return PMLG_OK;
@@ -899,6 +1021,52 @@ pmlg_status pmlg_batch_plan_profile(pmlg_batch_plan plan, int reps, double *ms)
     }
     for (int i = 0; i < 3; ++i)
         ms[i] = acc[i] / reps;
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_batch_plan_time_stages(pmlg_batch_plan plan, int ring)
+{
+    if (!plan || ring < 0 || ring > 4096)
+        return fail(PMLG_ERR_ARG, "batch_plan_time_stages: bad arguments");
+    if (!(plan->flags & PMLG_BATCH_GRAPH))
+        return fail(PMLG_ERR_ARG, "batch_plan_time_stages: the plan is not captured as a graph");
+    DeviceGuard g(plan->device);
+    (void)hipStreamSynchronize(plan->stream);
+    free_ring(plan);
+    if (plan->exec)
+        (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph)
+        (void)hipGraphDestroy(plan->graph);
+    plan->exec = nullptr;
+    plan->graph = nullptr;
+    plan->ring = ring;
+    return PMLG_OK;
+}
+
+pmlg_status pmlg_batch_plan_stage_times(pmlg_batch_plan plan, double *ms, int64_t *replays)
+{
+    if (!plan || !ms)
+        return fail(PMLG_ERR_ARG, "batch_plan_stage_times: bad arguments");
+    DeviceGuard g(plan->device);
+    double acc[3] = {0, 0, 0};
+    int64_t n = 0;
+    for (size_t r = 0; r < plan->rev.size(); ++r) {
+        if (plan->replays[r] == 0)
+            continue;
+        HIP_TRY(hipEventSynchronize(plan->rev[r][3]), "hipEventSynchronize");
+        for (int i = 0; i < 3; ++i) {
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, plan->rev[r][i], plan->rev[r][i + 1]),
+                    "hipEventElapsedTime");
+            acc[i] += t;
+        }
+        ++n;
+        plan->replays[r] = 0;
+    }
+    for (int i = 0; i < 3; ++i)
+        ms[i] = n ? acc[i] / (double)n : 0.0;
+    if (replays)
+        *replays = n;
     return PMLG_OK;
 }
 

@@ -1,10 +1,11 @@
-// Products modulo any p in [2, 2^62) whose roots of unity do not reach the
-// transform length (non-FFT primes, composite moduli): the integer product of
+// Products modulo any p in [2, 2^64) whose roots of unity do not reach the
+// transform length (non-FFT primes, composite moduli, every p >= 2^62): the
+// integer product of
 // the residues is computed modulo three FFT primes q1 > q2 > q3 in (2^60, 2^61)
 // with the ordinary device path, recombined by Garner's mixed-radix CRT and
 // reduced mod p.  A coefficient of A*B is at most k min(lenA, lenB) (p-1)^2 <
-// 2^124 k min(lenA, lenB), below q1 q2 q3 > 2^180 for every size the API
-// accepts, so the recombined integer is the exact coefficient.  This is the
+// 2^128 k min(lenA, lenB), below q1 q2 q3 > 2^180 for k min(lenA, lenB) < 2^52,
+// so the recombined integer is the exact coefficient.  This is the
 // multi-prime scheme of ntl-extras (mat_lzz_pX_multiply_3_primes.cpp:47-200),
 // here on the GPU transforms.
 #include <string>
@@ -27,9 +28,10 @@ struct CrtConst {
     uint64_t i123, i123q;    // (q1 q2)^-1 mod q3
     uint64_t q1_3, q1_3q;    // q1 mod q3
     uint64_t q1_p, q12_p;    // q1 mod p, q1 q2 mod p
+    int big;                 // p >= 2^62: reductions for any 64-bit p
 };
 
-__device__ __forceinline__ uint64_t mulmod_any(uint64_t a, uint64_t b, const ModConst &mc)
+__device__ __forceinline__ uint64_t mulmod_p(uint64_t a, uint64_t b, const ModConst &mc)
 {
     uint64_t hi, lo;
     mul128(a, b, hi, lo);
@@ -59,11 +61,18 @@ crt_kernel(uint64_t *__restrict__ out, int64_t out_stride, const uint64_t *__res
         t = c3[idx] + (u > c3[idx] ? k.q3 : 0) - u;
         const uint64_t v3 = shoup<uint64_t>(t, k.i123, k.i123q, k.q3);
         // x = v1 + q1 v2 + q1 q2 v3, reduced mod p
-        uint64_t x = barrett(v1, k.mp.bm, k.p);
-        x += mulmod_any(k.q1_p, v2, k.mp);
-        x = x >= k.p ? x - k.p : x;
-        x += mulmod_any(k.q12_p, v3, k.mp);
-        x = x >= k.p ? x - k.p : x;
+        uint64_t x;
+        if (k.big) {
+            // p >= 2^62: v1, v2, v3 < 2^61 are already below p; sums may carry
+            x = addmod_any(v1, mulmod_any(k.q1_p, v2, k.mp), k.p);
+            x = addmod_any(x, mulmod_any(k.q12_p, v3, k.mp), k.p);
+        } else {
+            x = barrett(v1, k.mp.bm, k.p);
+            x += mulmod_p(k.q1_p, v2, k.mp);
+            x = x >= k.p ? x - k.p : x;
+            x += mulmod_p(k.q12_p, v3, k.mp);
+            x = x >= k.p ? x - k.p : x;
+        }
         out[e * out_stride + d] = x;
     }
 }
@@ -103,6 +112,9 @@ pmlg_status crt_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
             if (st != PMLG_OK)
                 return st;
         }
+    if ((unsigned __int128)k * (uint64_t)(la < lb ? la : lb) >= ((unsigned __int128)1 << 52))
+        return fail(PMLG_ERR_UNSUPPORTED, "multi-prime path: k * min(lenA, lenB) must stay below "
+                                          "2^52 for the three-prime recombination to be exact");
     const int64_t lc = la + lb - 1;
     const int64_t need = hi > lc - nlo ? hi : lc - nlo;
     if (ceil_log2(need) > ctx->crt[0]->max_log)
@@ -181,6 +193,7 @@ pmlg_status crt_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
     c.q1_3q = host_shoup_pre64(c.q1_3, q3);
     c.q1_p = q1 % p;
     c.q12_p = host_mulmod(q1 % p, q2 % p, p);
+    c.big = p >= ((uint64_t)1 << 62) ? 1 : 0;
     hipLaunchKernelGGL(crt_kernel, dim3(crt_grid((int64_t)outw)), dim3(256), 0, s, dOut, out_stride,
                        cres, cres + outw, cres + 2 * outw, width, m * n, c);
     HIP_TRY(hipGetLastError(), "crt_kernel");

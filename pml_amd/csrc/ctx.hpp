@@ -47,6 +47,10 @@ struct Tables {
     void *mem = nullptr;
     NttTab<uint64_t> t64{};
     NttTab<uint32_t> t32{};
+    // long transforms (N above 4 times the LDS limit): column-pass tables;
+    // the full tables (tw, twq, itw, itwq) are not built for them
+    ColTab<uint64_t> c64{};
+    ColTab<uint32_t> c32{};
 };
 
 // Pinned double-buffered staging chunks with their copy stream, one per

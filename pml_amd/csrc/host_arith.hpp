@@ -139,6 +139,9 @@ static inline ModConst host_modconst(uint64_t p)
         mc.limbs = 0;  // MFMA path unavailable
     mc.p32 = (uint32_t)p;
     mc.p32x2 = (uint32_t)(2 * p);
+    mc.norm = __builtin_clzll(p);
+    mc.dnorm = p << mc.norm;
+    mc.dinv = (uint64_t)((~(u128_t)0) / mc.dnorm - ((u128_t)1 << 64));
     return mc;
 }
 

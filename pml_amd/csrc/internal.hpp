@@ -21,16 +21,35 @@ struct NttTab {
     W ninvr, ninvrq;  // N^-1 * 2^64 mod p: scaling after a Montgomery (REDC) pointwise product
 };
 
-// Largest transform held in LDS by one workgroup (elements).
+// Tables of the long transforms (N = R M with R > 4): the column pass runs
+// R-point transforms over the M columns of each entry, then multiplies
+// column i1, output g by w^(i1 g).  rt[k] = w_R^k and rti[k] = w_R^-k for
+// k < R/2 (w_R = w^M); w^a = lo[a mod 2^slog] * hi[a >> slog] (and the same
+// for w^-a with loi, hii), so the twist needs 2 sqrt(N) words, not N.
+template <typename W>
+struct ColTab {
+    const W *rt, *rtq, *rti, *rtiq;
+    const W *lo, *loq, *hi, *hiq;
+    const W *loi, *loiq, *hii, *hiiq;
+    int slog;
+};
+
+// Largest transform held in LDS by one workgroup (elements), and the
+// largest transform at all: the column pass holds R x 4 words of one entry
+// in 64 KiB of LDS.
 template <typename W>
 struct NttLimits;
 template <>
 struct NttLimits<uint64_t> {
     static constexpr int max_log_m = 13;  // 64 KiB of LDS
+    static constexpr int max_log_r = 11;  // 2^11 x 4 columns x 8 bytes = 64 KiB
+    static constexpr int max_log_n = max_log_m + max_log_r;
 };
 template <>
 struct NttLimits<uint32_t> {
     static constexpr int max_log_m = 14;  // 64 KiB of LDS
+    static constexpr int max_log_r = 12;  // 2^12 x 4 columns x 4 bytes = 64 KiB
+    static constexpr int max_log_n = max_log_m + max_log_r;
 };
 
 // Tuning knobs of the transforms (picked per context, see the autotuner).
@@ -44,18 +63,23 @@ struct NttTune {
 
 // Forward transforms of `count` entries: entry e reads in[e*in_stride + i]
 // for i < in_len (zero beyond), writes its N evaluations in blocked layout.
+// Transforms longer than 4 times the LDS limit run a column pass first into
+// tmp (count*N words of Win) with the tables ct.
 template <typename Win, typename W>
 hipError_t launch_ntt_fwd(const Win *in, int64_t in_stride, int64_t in_len, W *out,
                           int64_t count, int logN, int logP, const NttTune &tune,
-                          const NttTab<W> &tab, W p, hipStream_t s);
+                          const NttTab<W> &tab, W p, hipStream_t s, Win *tmp = nullptr,
+                          const ColTab<W> *ct = nullptr);
 
 // Inverse transforms (blocked input): coefficients [lo, hi) of each entry
 // (clipped to N), scaled by 1/N, written to out[e*out_stride + (i - lo)] as
-// Wout in [0,p).  tmp must hold count*N words of W when N exceeds the LDS limit.
+// Wout in [0,p).  tmp must hold count*N words of W when N exceeds the LDS
+// limit; longer than 4 times that limit, the column pass uses ct.
 template <typename W, typename Wout>
 hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
                           int64_t count, int logN, int logP, const NttTune &tune,
-                          const NttTab<W> &tab, W p, W *tmp, hipStream_t s);
+                          const NttTab<W> &tab, W p, W *tmp, hipStream_t s,
+                          const ColTab<W> *ct = nullptr);
 
 // ---------------------------------------------------------------------------
 // Segment tables of the batched launches.  One launch serves many operand

@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(256) entry_degree_kernel(const uint64_t *__res
         deg[e] = result;
 }
 
-constexpr int kMaxRows = 128;
+constexpr int kMaxRows = 256;  // rows of F (the approximant bases' limit)
+constexpr int kMaxApproxCols = 128;  // columns of one approximant basis
 
 // Row moves of one launch: destination row dst[t] <- source row src[t].
 struct RowMap {
@@ -465,7 +466,7 @@ pmlg_status via_approx(Kx &X, const DMat &Fin, int64_t *shift, int64_t *pivind, 
         *nul = 0;
         return PMLG_OK;
     }
-    if (n > kMaxRows)
+    if (n > kMaxApproxCols)
         return fail(PMLG_ERR_ARG, "kernel basis via approximation: at most 128 columns");
     if (m <= n) {
         // full row rank of the constant coefficient means an empty kernel
@@ -539,10 +540,10 @@ pmlg_status check_kernel_args(pmlg_ctx ctx, int64_t m, int64_t n, int64_t lenF, 
                               const int64_t *lenK)
 {
     if (m < 1 || m > kMaxRows || n < 1 || n > (1 << 20) || lenF < 0)
-        return fail(PMLG_ERR_ARG, "kernel basis: 1 <= rows <= 128, columns >= 1, lenF >= 0");
+        return fail(PMLG_ERR_ARG, "kernel basis: 1 <= rows <= 256, columns >= 1, lenF >= 0");
     if (algo != PMLG_KERNEL_ZLS && algo != PMLG_KERNEL_VIA_APPROX)
         return fail(PMLG_ERR_ARG, "kernel basis: unknown algorithm");
-    if (algo == PMLG_KERNEL_VIA_APPROX && n > kMaxRows)
+    if (algo == PMLG_KERNEL_VIA_APPROX && n > kMaxApproxCols)
         return fail(PMLG_ERR_ARG, "kernel basis via approximation: at most 128 columns");
     if (!shift || !pivind || !nullity || !lenK)
         return fail(PMLG_ERR_ARG, "kernel basis: null output pointer");

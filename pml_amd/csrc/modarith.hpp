@@ -114,6 +114,10 @@ struct ModConst {
     uint32_t p32, p32x2, p32q_unused;
     int bits;           // bit length of p
     int limbs;          // 8 for p < 2^62, 4 for p < 2^30
+    // any p < 2^64: division of 128-bit values by the normalised modulus
+    int norm;           // leading zero bits of p
+    uint64_t dnorm;     // p << norm
+    uint64_t dinv;      // floor((2^128 - 1) / dnorm) - 2^64
 };
 
 // Reduce a 128-bit value (hi, lo), hi < 2^64, to [0, p).
@@ -146,6 +150,44 @@ PMLG_HD void mul128(uint64_t a, uint64_t b, uint64_t &hi, uint64_t &lo)
     hi = (uint64_t)(r >> 64);
     lo = (uint64_t)r;
 #endif
+}
+
+// (hi 2^64 + lo) mod p for any p < 2^64 and hi < p: the value and the
+// modulus shifted left by `norm` (the top bit of the divisor set), then one
+// division step with the precomputed reciprocal of Moller and Granlund
+// ("Improved division by invariant integers", IEEE Trans. Computers 60(2),
+// 2011, Algorithm 4), whose remainder is shifted back.
+PMLG_HD uint64_t mod128_any(uint64_t hi, uint64_t lo, const ModConst &mc)
+{
+    const uint64_t u1 = mc.norm ? (hi << mc.norm) | (lo >> (64 - mc.norm)) : hi;
+    const uint64_t u0 = lo << mc.norm;
+    const uint64_t d = mc.dnorm;
+    uint64_t q1, q0;
+    mul128(mc.dinv, u1, q1, q0);
+    q0 += u0;
+    q1 += u1 + (q0 < u0 ? 1 : 0);
+    q1 += 1;
+    uint64_t r = u0 - q1 * d;
+    if (r > q0)
+        r += d;
+    if (r >= d)
+        r -= d;
+    return r >> mc.norm;
+}
+
+// a * b mod p for residues a, b < p, any p < 2^64
+PMLG_HD uint64_t mulmod_any(uint64_t a, uint64_t b, const ModConst &mc)
+{
+    uint64_t hi, lo;
+    mul128(a, b, hi, lo);
+    return mod128_any(hi, lo, mc);
+}
+
+// a + b mod p for residues a, b < p, any p < 2^64 (the sum may carry out)
+PMLG_HD uint64_t addmod_any(uint64_t a, uint64_t b, uint64_t p)
+{
+    const uint64_t s = a + b;
+    return (s < a || s >= p) ? s - p : s;
 }
 
 // 64x64 -> 128 multiply-accumulate into (hi, lo)

@@ -619,11 +619,13 @@ __device__ __forceinline__ void inv_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 <<
     }
 }
 
+// pre: the input rows are the column pass's outputs (row g of entry e at
+// in + e*in_stride + g*M, every word valid): no radix-R step on the load.
 template <typename Win, typename W, bool LOOSE, int L, int LOGE>
 __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t in_stride,
                                              int64_t in_len, W *__restrict__ out, int64_t units,
                                              int64_t u, int logR, int logP, const NttTab<W> &tab,
-                                             W p)
+                                             W p, bool pre = false)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
@@ -645,7 +647,9 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
     for (int j = 0; j < E; ++j) {
         const int i1 = (j << (L - LOGE)) | t;
         W v = 0;
-        if (R == 1) {
+        if (pre) {
+            v = (W)xin[(int64_t)gg * M + i1];
+        } else if (R == 1) {
             if (i1 < in_len)
                 v = (W)xin[i1];
         } else {
@@ -695,10 +699,10 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
 template <typename Win, typename W, bool LOOSE, int L, int LOGE>
 __global__ void __launch_bounds__(1 << (L - LOGE))
 ntt_fwd_reg(const Win *__restrict__ in, int64_t in_stride, int64_t in_len, W *__restrict__ out,
-            int64_t units, int logR, int logP, NttTab<W> tab, W p)
+            int64_t units, int logR, int logP, NttTab<W> tab, W p, int pre)
 {
     fwd_reg_body<Win, W, LOOSE, L, LOGE>(in, in_stride, in_len, out, units,
-                                         xcd_unit(blockIdx.x, units), logR, logP, tab, p);
+                                         xcd_unit(blockIdx.x, units), logR, logP, tab, p, pre != 0);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
@@ -775,7 +779,7 @@ ntt_inv_reg(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride
 template <typename Win, typename W, bool LOOSE, int L, int LOGE>
 hipError_t launch_fwd_reg3(const Win *in, int64_t in_stride, int64_t in_len, W *out,
                            int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
-                           hipStream_t s)
+                           hipStream_t s, int pre)
 {
     constexpr int T = 1 << (L - LOGE);
     const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
@@ -788,7 +792,7 @@ hipError_t launch_fwd_reg3(const Win *in, int64_t in_stride, int64_t in_len, W *
         attr = true;
     }
     hipLaunchKernelGGL((ntt_fwd_reg<Win, W, LOOSE, L, LOGE>), dim3((unsigned)units), dim3(T), lds,
-                       s, in, in_stride, in_len, out, units, logR, logP, tab, p);
+                       s, in, in_stride, in_len, out, units, logR, logP, tab, p, pre);
     return hipGetLastError();
 }
 
@@ -815,13 +819,14 @@ hipError_t launch_inv_reg3(const W *in, Wout *out, int64_t out_stride, int64_t l
 template <typename Win, typename W, bool LOOSE, int L>
 hipError_t launch_fwd_reg2(int logE, const Win *in, int64_t in_stride, int64_t in_len, W *out,
                            int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
-                           hipStream_t s)
+                           hipStream_t s, int pre)
 {
     if (logE == 5 && L >= 12)
         return launch_fwd_reg3<Win, W, LOOSE, L, (L >= 12 ? 5 : 4)>(in, in_stride, in_len, out,
-                                                                   units, logR, logP, tab, p, s);
+                                                                   units, logR, logP, tab, p, s,
+                                                                   pre);
     return launch_fwd_reg3<Win, W, LOOSE, L, 4>(in, in_stride, in_len, out, units, logR, logP, tab,
-                                                p, s);
+                                                p, s, pre);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L>
@@ -840,15 +845,15 @@ hipError_t launch_inv_reg2(int logE, const W *in, Wout *out, int64_t out_stride,
 template <typename Win, typename W, int L>
 hipError_t launch_fwd_reg(int logE, const Win *in, int64_t in_stride, int64_t in_len, W *out,
                           int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
-                          hipStream_t s)
+                          hipStream_t s, int pre)
 {
     if constexpr (sizeof(W) == 8) {
         if ((uint64_t)p < ((uint64_t)1 << 61))
             return launch_fwd_reg2<Win, W, true, L>(logE, in, in_stride, in_len, out, units, logR,
-                                                    logP, tab, p, s);
+                                                    logP, tab, p, s, pre);
     }
     return launch_fwd_reg2<Win, W, false, L>(logE, in, in_stride, in_len, out, units, logR, logP,
-                                             tab, p, s);
+                                             tab, p, s, pre);
 }
 
 template <typename W, typename Wout, int L>
@@ -868,19 +873,22 @@ hipError_t launch_inv_reg(int logE, const W *in, Wout *out, int64_t out_stride, 
 template <typename Win, typename W>
 hipError_t dispatch_fwd_reg(int logM, int logE, const Win *in, int64_t in_stride, int64_t in_len,
                             W *out, int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
-                            hipStream_t s)
+                            hipStream_t s, int pre = 0)
 {
     switch (logM) {
     case 11:
-        return launch_fwd_reg<Win, W, 11>(logE, in, in_stride, in_len, out, units, logR, logP, tab, p, s);
+        return launch_fwd_reg<Win, W, 11>(logE, in, in_stride, in_len, out, units, logR, logP, tab,
+                                          p, s, pre);
     case 12:
-        return launch_fwd_reg<Win, W, 12>(logE, in, in_stride, in_len, out, units, logR, logP, tab, p, s);
+        return launch_fwd_reg<Win, W, 12>(logE, in, in_stride, in_len, out, units, logR, logP, tab,
+                                          p, s, pre);
     case 13:
-        return launch_fwd_reg<Win, W, 13>(logE, in, in_stride, in_len, out, units, logR, logP, tab, p, s);
+        return launch_fwd_reg<Win, W, 13>(logE, in, in_stride, in_len, out, units, logR, logP, tab,
+                                          p, s, pre);
     case 14:
         if constexpr (sizeof(W) == 4)
             return launch_fwd_reg<Win, W, 14>(logE, in, in_stride, in_len, out, units, logR, logP,
-                                              tab, p, s);
+                                              tab, p, s, pre);
         [[fallthrough]];
     default:
         return hipErrorInvalidValue;
@@ -912,6 +920,161 @@ hipError_t dispatch_inv_reg(int logM, int logE, const W *in, Wout *out, int64_t 
     }
 }
 
+// ===========================================================================
+// Column pass of the long transforms, N = R M with R > 4 (M the LDS limit).
+// X[R k + g] = sum_i1 w_M^(i1 k) y_g[i1] with y_g[i1] = w^(i1 g) sum_t
+// x[i1 + M t] w_R^(t g): the column pass forms every y_g (one R-point
+// transform per column i1, then the twist w^(i1 g)) into rows of M words,
+// and the register kernels transform the rows.  The inverse runs the rows'
+// M-point inverses into tmp, then x[i1 + M t] = N^-1 sum_g w_R^(-t g)
+// w^(-i1 g) z_g[i1] here.  One workgroup holds 2^logC consecutive columns
+// of one entry (all R rows) in LDS, [t][c], values in [0, 2p).
+// ===========================================================================
+constexpr int kColThreads = 256;
+
+__device__ __forceinline__ int bitrev(int j, int bits)
+{
+    return (int)(__builtin_bitreverse32((uint32_t)j) >> (32 - bits));
+}
+
+// w^a for a < N from the two-level tables, times x in [0, 2p): [0, 2p)
+template <typename W>
+__device__ __forceinline__ W twist(W x, int64_t a, const W *lo, const W *loq, const W *hi,
+                                   const W *hiq, int slog, W p)
+{
+    const int64_t al = a & (((int64_t)1 << slog) - 1), ah = a >> slog;
+    return shoup_lazy(shoup_lazy(x, lo[al], loq[al], p), hi[ah], hiq[ah], p);
+}
+
+template <typename Win, typename W>
+__global__ void __launch_bounds__(kColThreads)
+ntt_cols_fwd(const Win *__restrict__ in, int64_t in_stride, int64_t in_len, Win *__restrict__ tmp,
+             int logM, int logR, int logC, ColTab<W> ct, W p)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    W *s = reinterpret_cast<W *>(smem_raw);
+    const int C = 1 << logC, R = 1 << logR;
+    const int64_t M = (int64_t)1 << logM, N = M << logR;
+    const int64_t groups = M >> logC;  // column groups per entry
+    const int64_t e = blockIdx.x / groups;
+    const int64_t c0 = (blockIdx.x - e * groups) << logC;
+    const Win *x = in + e * in_stride;
+    const W p2 = p + p;
+    for (int idx = threadIdx.x; idx < R * C; idx += kColThreads) {
+        const int64_t src = c0 + (idx & (C - 1)) + M * (idx >> logC);
+        s[idx] = src < in_len ? (W)x[src] : (W)0;
+    }
+    __syncthreads();
+    // R-point decimation in frequency over the rows: natural in, slot j
+    // holds output g = bitrev(j) afterwards
+    for (int lh = logR - 1; lh >= 0; --lh) {
+        const int h = 1 << lh;
+        for (int b = threadIdx.x; b < (R / 2) * C; b += kColThreads) {
+            const int c = b & (C - 1), j = b >> logC;
+            const int jj = j & (h - 1);
+            const int i0 = (((j >> lh) << (lh + 1)) + jj) * C + c;
+            const int i1 = i0 + h * C;
+            const int k = jj << (logR - 1 - lh);
+            const W u = s[i0], v = s[i1];
+            s[i0] = add_2p<W>(u, v, p2);
+            s[i1] = shoup_lazy(sub_2p<W>(u, v, p2), ct.rt[k], ct.rtq[k], p);
+        }
+        __syncthreads();
+    }
+    Win *y = tmp + e * N;
+    for (int idx = threadIdx.x; idx < R * C; idx += kColThreads) {
+        const int c = idx & (C - 1);
+        const int g = bitrev(idx >> logC, logR);
+        const int64_t i1 = c0 + c;
+        const W v = twist<W>(s[idx], i1 * g, ct.lo, ct.loq, ct.hi, ct.hiq, ct.slog, p);
+        y[(int64_t)g * M + i1] = (Win)reduce_2p<W>(v, p);
+    }
+}
+
+template <typename W, typename Wout>
+__global__ void __launch_bounds__(kColThreads)
+ntt_cols_inv(const W *__restrict__ tmp, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
+             int64_t hi, int logM, int logR, int logC, ColTab<W> ct, W ninv, W ninvq, W p)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    W *s = reinterpret_cast<W *>(smem_raw);
+    const int C = 1 << logC, R = 1 << logR;
+    const int64_t M = (int64_t)1 << logM, N = M << logR;
+    const int64_t groups = M >> logC;
+    const int64_t e = blockIdx.x / groups;
+    const int64_t c0 = (blockIdx.x - e * groups) << logC;
+    const W *z = tmp + e * N;
+    const W p2 = p + p;
+    // slot j <- w^(-i1 g) z_g[i1] with g = bitrev(j): bit-reversed order in
+    for (int idx = threadIdx.x; idx < R * C; idx += kColThreads) {
+        const int c = idx & (C - 1);
+        const int g = bitrev(idx >> logC, logR);
+        const int64_t i1 = c0 + c;
+        s[idx] = twist<W>(z[(int64_t)g * M + i1], i1 * g, ct.loi, ct.loiq, ct.hii, ct.hiiq, ct.slog,
+                          p);
+    }
+    __syncthreads();
+    // R-point decimation in time with w_R^-1: natural order out
+    for (int lh = 0; lh < logR; ++lh) {
+        const int h = 1 << lh;
+        for (int b = threadIdx.x; b < (R / 2) * C; b += kColThreads) {
+            const int c = b & (C - 1), j = b >> logC;
+            const int jj = j & (h - 1);
+            const int i0 = (((j >> lh) << (lh + 1)) + jj) * C + c;
+            const int i1 = i0 + h * C;
+            const int k = jj << (logR - 1 - lh);
+            const W u = s[i0];
+            const W v = shoup_lazy(s[i1], ct.rti[k], ct.rtiq[k], p);
+            s[i0] = add_2p<W>(u, v, p2);
+            const W d = sub_2p<W>(u, v, p2);
+            s[i1] = d >= p2 ? d - p2 : d;
+        }
+        __syncthreads();
+    }
+    Wout *o = out + e * out_stride;
+    for (int idx = threadIdx.x; idx < R * C; idx += kColThreads) {
+        const int64_t pos = c0 + (idx & (C - 1)) + M * (idx >> logC);
+        if (pos >= lo && pos < hi)
+            o[pos - lo] = (Wout)shoup<W>(s[idx], ninv, ninvq, p);
+    }
+}
+
+// columns per workgroup: 16 (128-byte runs of 64-bit words) while R x 16
+// words fit 64 KiB of LDS (the default limit of dynamic LDS), fewer for the
+// largest R
+template <typename W>
+int col_log_c(int logR)
+{
+    const int room = 16 - (sizeof(W) == 8 ? 3 : 2) - logR;  // log2(65536 / (R * sizeof(W)))
+    return room < 2 ? 2 : (room > 4 ? 4 : room);
+}
+
+template <typename Win, typename W>
+hipError_t launch_cols_fwd(const Win *in, int64_t in_stride, int64_t in_len, Win *tmp,
+                           int64_t count, int logM, int logR, const ColTab<W> &ct, W p,
+                           hipStream_t s)
+{
+    const int logC = col_log_c<W>(logR);
+    const size_t lds = ((size_t)1 << (logR + logC)) * sizeof(W);
+    const int64_t blocks = count << (logM - logC);
+    hipLaunchKernelGGL((ntt_cols_fwd<Win, W>), dim3((unsigned)blocks), dim3(kColThreads), lds, s, in,
+                       in_stride, in_len, tmp, logM, logR, logC, ct, p);
+    return hipGetLastError();
+}
+
+template <typename W, typename Wout>
+hipError_t launch_cols_inv(const W *tmp, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
+                           int64_t count, int logM, int logR, const ColTab<W> &ct, W ninv,
+                           W ninvq, W p, hipStream_t s)
+{
+    const int logC = col_log_c<W>(logR);
+    const size_t lds = ((size_t)1 << (logR + logC)) * sizeof(W);
+    const int64_t blocks = count << (logM - logC);
+    hipLaunchKernelGGL((ntt_cols_inv<W, Wout>), dim3((unsigned)blocks), dim3(kColThreads), lds, s,
+                       tmp, out, out_stride, lo, hi, logM, logR, logC, ct, ninv, ninvq, p);
+    return hipGetLastError();
+}
+
 constexpr int kRegMinLog = 11;
 
 }  // namespace
@@ -919,15 +1082,27 @@ constexpr int kRegMinLog = 11;
 template <typename Win, typename W>
 hipError_t launch_ntt_fwd(const Win *in, int64_t in_stride, int64_t in_len, W *out,
                           int64_t count, int logN, int logP, const NttTune &tune,
-                          const NttTab<W> &tab, W p, hipStream_t s)
+                          const NttTab<W> &tab, W p, hipStream_t s, Win *tmp,
+                          const ColTab<W> *ct)
 {
     if (count <= 0)
         return hipSuccess;
     const int logM = logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
     const int logR = logN - logM;
-    if (logR > 2 || logP > logN)
+    if (logR > NttLimits<W>::max_log_r || logP > logN)
         return hipErrorInvalidValue;
     const int64_t units = count << logR;
+    if (logR > 2) {
+        // long transform: column pass into tmp, then the rows
+        if (!tmp || !ct)
+            return hipErrorInvalidValue;
+        hipError_t e = launch_cols_fwd<Win, W>(in, in_stride, in_len, tmp, count, logM, logR, *ct,
+                                               p, s);
+        if (e != hipSuccess)
+            return e;
+        return dispatch_fwd_reg<Win, W>(logM, tune.fwd_log_e, tmp, (int64_t)1 << logN,
+                                        (int64_t)1 << logN, out, units, logR, logP, tab, p, s, 1);
+    }
     if (logM >= kRegMinLog)
         return dispatch_fwd_reg<Win, W>(logM, tune.fwd_log_e, in, in_stride, in_len, out, units,
                                         logR, logP, tab, p, s);
@@ -942,7 +1117,7 @@ hipError_t launch_ntt_fwd(const Win *in, int64_t in_stride, int64_t in_len, W *o
 template <typename W, typename Wout>
 hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
                           int64_t count, int logN, int logP, const NttTune &tune,
-                          const NttTab<W> &tab, W p, W *tmp, hipStream_t s)
+                          const NttTab<W> &tab, W p, W *tmp, hipStream_t s, const ColTab<W> *ct)
 {
     const int64_t N = (int64_t)1 << logN;
     if (hi > N)
@@ -951,7 +1126,8 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
         return hipSuccess;
     const int logM = logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
     const int logR = logN - logM;
-    if (logR > 2 || (logR > 0 && tmp == nullptr) || logP > logN)
+    if (logR > NttLimits<W>::max_log_r || (logR > 0 && tmp == nullptr) || logP > logN ||
+        (logR > 2 && (!ct || logM < kRegMinLog)))
         return hipErrorInvalidValue;
     const int64_t units = count << logR;
     hipError_t err;
@@ -969,6 +1145,9 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
     }
     if (err != hipSuccess || logR == 0)
         return err;
+    if (logR > 2)
+        return launch_cols_inv<W, Wout>(tmp, out, out_stride, lo, hi, count, logM, logR, *ct,
+                                        tab.ninv, tab.ninvq, p, s);
     const int64_t total = count * (hi - lo);
     int64_t mblocks = (total + kThreads - 1) / kThreads;
     if (mblocks > 65536)
@@ -1283,18 +1462,22 @@ template hipError_t launch_ntt_inv_segs<uint32_t, uint64_t>(const InvSeg<uint32_
 template hipError_t launch_ntt_fwd<uint64_t, uint64_t>(const uint64_t *, int64_t, int64_t,
                                                        uint64_t *, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint64_t> &,
-                                                       uint64_t, hipStream_t);
+                                                       uint64_t, hipStream_t, uint64_t *,
+                                                       const ColTab<uint64_t> *);
 template hipError_t launch_ntt_fwd<uint64_t, uint32_t>(const uint64_t *, int64_t, int64_t,
                                                        uint32_t *, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint32_t> &,
-                                                       uint32_t, hipStream_t);
+                                                       uint32_t, hipStream_t, uint64_t *,
+                                                       const ColTab<uint32_t> *);
 template hipError_t launch_ntt_inv<uint64_t, uint64_t>(const uint64_t *, uint64_t *, int64_t,
                                                        int64_t, int64_t, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint64_t> &,
-                                                       uint64_t, uint64_t *, hipStream_t);
+                                                       uint64_t, uint64_t *, hipStream_t,
+                                                       const ColTab<uint64_t> *);
 template hipError_t launch_ntt_inv<uint32_t, uint64_t>(const uint32_t *, uint64_t *, int64_t,
                                                        int64_t, int64_t, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint32_t> &,
-                                                       uint32_t, uint32_t *, hipStream_t);
+                                                       uint32_t, uint32_t *, hipStream_t,
+                                                       const ColTab<uint32_t> *);
 
 }  // namespace pmlg

@@ -45,8 +45,23 @@ namespace pmlg {
 namespace {
 
 constexpr int kStepThreads = 1024;
-constexpr int kMaxDim = 128;  // m, n limits of the device M-Basis step
+constexpr int kMaxM = 256;    // rows of F (basis dimension) in the device M-Basis step
+constexpr int kMaxN = 128;    // columns of F: at most two rows of the transposed residual per lane
 constexpr int kPanel = 4;     // widest Gauss-Jordan panel (LDS is sized for it)
+// dynamic LDS of the step kernel, set once for every instance: the value the
+// 128 x 128 shape needs, which every allowed shape must fit
+constexpr size_t kStepLdsMax = 147968;
+
+// LDS bytes of the step kernel for an m x n residual with words of wbytes:
+// shifts (m 64-bit words), At (n rows of odd stride m | 1), the pivot
+// cofactors (kMaxN), two coefficient buffers (kMaxN x (kPanel + 1) each),
+// then the int arrays (4 of m, 3 of kMaxN); as laid out in the kernel
+__host__ __device__ inline size_t step_lds_bytes(int m, int n, size_t wbytes)
+{
+    return (size_t)m * 8 +
+           ((size_t)n * (m | 1) + kMaxN + 2 * (size_t)kMaxN * (kPanel + 1)) * wbytes +
+           (4 * (size_t)m + 3 * (size_t)kMaxN) * sizeof(int);
+}
 constexpr int kXPad = 32;     // words after X read (and discarded) by the last update group
 
 // control words of one leaf
@@ -88,6 +103,8 @@ __device__ __forceinline__ uint64_t mulmod_g(uint64_t a, uint64_t b, const ModCo
 {
     if (mc.p < ((uint64_t)1 << 32))
         return barrett(a * b, mc.bm, mc.p);
+    if (mc.bits > 62)
+        return mulmod_any(a, b, mc);
     uint64_t hi, lo;
     mul128(a, b, hi, lo);
     return reduce128(hi, lo, mc);
@@ -287,6 +304,45 @@ struct Mont32Arith {
     }
 };
 
+// Moduli from 2^62 to 2^64 (odd primes): plain residues, every product
+// reduced at once by the division of mod128_any (Moller-Granlund); sums of
+// residues may carry out of 64 bits, so additions go through addmod_any.
+// Slower than the Montgomery forms, which need p < 2^62 for their lazy sums.
+struct BigArith {
+    using V = uint64_t;
+    ModConst mc;
+    __device__ explicit BigArith(const ModConst &c) : mc(c) {}
+    __device__ V mul(V a, V b) const { return mulmod_any(a, b, mc); }
+    using Acc = uint64_t;  // a reduced running sum
+    __device__ Acc prod(V a, V b) const { return mul(a, b); }
+    __device__ Acc fma(Acc t, V a, V b) const { return addmod_any(t, mul(a, b), mc.p); }
+    __device__ V reduce(Acc t) const { return t; }
+    // pv x - fr q
+    __device__ V msub(V pv, V x, V fr, V q) const
+    {
+        return addmod_any(mul(pv, x), mul(fr, q ? mc.p - q : 0), mc.p);
+    }
+    __device__ V sub(V a, V b) const { return a >= b ? a - b : a + (mc.p - b); }
+    __device__ V add(V a, V b) const { return addmod_any(a, b, mc.p); }
+    __device__ V one() const { return 1; }
+    __device__ V to(uint64_t x) const { return x; }
+    __device__ uint64_t from(V x) const { return x; }
+    static __device__ V load(const uint64_t *w) { return *w; }
+    __device__ V inv(V a) const
+    {
+        // Fermat a^(p-2)
+        uint64_t e = mc.p - 2;
+        V r = 1, b = a;
+        while (e) {
+            if (e & 1)
+                r = mul(r, b);
+            b = mul(b, b);
+            e >>= 1;
+        }
+        return r;
+    }
+};
+
 // workgroup barrier ordering LDS only: global loads in flight stay in flight
 __device__ __forceinline__ void lds_barrier()
 {
@@ -336,17 +392,18 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     const Arith ar(a.mc);
     const int ms = m | 1;                   // odd row stride of At
     // 64-bit words first (sh), then the V arrays, then the int arrays
-    int64_t *sh = reinterpret_cast<int64_t *>(smem);             // kMaxDim
-    V *At = reinterpret_cast<V *>(sh + kMaxDim);                 // n x ms
-    V *cof = At + n * ms;                                        // kMaxDim: pivot cofactors
-    V *Ck = cof + kMaxDim;                                            // 2 x kMaxDim x (P + 1)
-    int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxDim * (P + 1));  // kMaxDim
-    int *pivcol = perm + kMaxDim;                                // kMaxDim
-    int *plist = pivcol + kMaxDim;                               // kMaxDim
-    int *prow = plist + kMaxDim;                                 // kMaxDim
-    int *ispiv = prow + kMaxDim;                                 // kMaxDim (rows of At)
-    int *colpiv = ispiv + kMaxDim;                               // kMaxDim (columns of At)
-    int *pos = colpiv + kMaxDim;                                 // kMaxDim
+    // (layout and sizes as step_lds_bytes, whose Ck is sized for kPanel)
+    int64_t *sh = reinterpret_cast<int64_t *>(smem);                 // m
+    V *At = reinterpret_cast<V *>(sh + m);                           // n x ms
+    V *cof = At + n * ms;                                            // kMaxN: pivot cofactors
+    V *Ck = cof + kMaxN;                                             // 2 x kMaxN x (P + 1)
+    int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxN * (kPanel + 1));  // m
+    int *plist = perm + m;                                           // m
+    int *colpiv = plist + m;                                         // m (columns of At)
+    int *pos = colpiv + m;                                           // m
+    int *pivcol = pos + m;                                           // kMaxN (by rank)
+    int *prow = pivcol + kMaxN;                                      // kMaxN (by rank)
+    int *ispiv = prow + kMaxN;                                       // kMaxN (rows of At)
     __shared__ int s_nb[2], s_mask[2], s_prs[2][kPanel];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -359,7 +416,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // stored to LDS before those loads are issued (a wait on a vector load
     // also waits on those issued before it).  Thread 0 writes the control
     // words back only after full barriers.
-    const int64_t shv = tid < m ? a.shift[tid] : 0;  // m <= kMaxDim <= kStepThreads
+    const int64_t shv = tid < m ? a.shift[tid] : 0;  // m <= kMaxM <= kStepThreads
     int *c = a.ctl;
     const int flip = c[C_FLIP], cur0 = c[C_CUR], done = c[C_DONE];
     if (tid < m) {
@@ -380,7 +437,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         const int q = tid + k * kStepThreads;
         pre[k] = Arith::load(a.Rn + (q < mn ? q : mn - 1));
     }
-    for (int r = tid; r < kMaxDim; r += kStepThreads)
+    for (int r = tid; r < kMaxN; r += kStepThreads)
         ispiv[r] = 0;
     if (done) {
         __syncthreads();
@@ -533,7 +590,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 if (ok[h]) {
 #pragma unroll
                     for (int s2 = 0; s2 <= P; ++s2)
-                        Ck[(buf * kMaxDim + r) * (P + 1) + s2] = cc[h][s2];
+                        Ck[(buf * kMaxN + r) * (P + 1) + s2] = cc[h][s2];
                     ispiv[r] = isp[h] ? 1 : 0;
                 }
             }
@@ -559,7 +616,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
 #pragma unroll
         for (int q = 0; q < P; ++q)
             pvr[q] = mask & (1 << q) ? At[s_prs[buf][q] * ms + cg] : (V)0;
-        const V *C = Ck + buf * kMaxDim * (P + 1);
+        const V *C = Ck + buf * kMaxN * (P + 1);
         for (int r = g; r < n; r += G) {
             const V *cr = C + r * (P + 1);
             V *x = At + r * ms + cg;
@@ -630,7 +687,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         const int l = tid - (kStepThreads - 64);
         int base = 0;
 #pragma unroll
-        for (int h = 0; h < kMaxDim / 64; ++h) {
+        for (int h = 0; h < kMaxM / 64; ++h) {
             const int i = l + 64 * h;
             const bool pc = i < m && colpiv[i];
             const unsigned long long b = __ballot(pc);
@@ -699,9 +756,10 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
 // loads of a group of terms (32 words below 2^30, or 16 64-bit words: one L2
 // round trip) are issued before its products, with no branch between them:
 // terms past the rank read row piv[r] of the LDS row list (padded with row 0
-// up to kMaxDim) and X words past the row (X carries kXPad words of padding),
+// up to kMaxM; rank <= kMaxN) and X words past the row (X carries kXPad words
+// of padding),
 // and their X coefficient is zeroed by a select after the load.
-template <bool SMALL>
+template <bool SMALL, bool BIG>
 __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArgs &a, const int *piv, int t,
                                               int rank, int stride, int cols, int j, int d, uint64_t base)
 {
@@ -730,6 +788,18 @@ __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArg
                     part += (uint64_t)xv[u] * sv[u];
                 acc = barrett(acc + part, a.mc.bm, a.mc.p);
             }
+        } else if constexpr (BIG) {
+            // p >= 2^62: every product reduced at once, the sum kept below p
+            uint64_t sv[G], xv[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                sv[u] = sj[(size_t)piv[r + u] * rstride];
+                const uint64_t x = xr[r + u];
+                xv[u] = r + u < rank ? x : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                acc = addmod_any(acc, mulmod_any(xv[u], sv[u], a.mc), a.mc.p);
         } else {
             // 64-bit words (p < 2^62): an exact 128-bit sum of 16 products
             // and a residue, 16 (p - 1)^2 + p < 2^128 - 2^67, folded once (the
@@ -751,7 +821,7 @@ __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArg
     return acc;
 }
 
-template <bool SMALL>
+template <bool SMALL, bool BIG>
 __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
 {
     const int mode = a.ctl[C_MODE];
@@ -782,9 +852,9 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
         }
         return;
     }
-    __shared__ int piv[kMaxDim], rinfo[kMaxDim];
-    for (int i = threadIdx.x; i < kMaxDim; i += blockDim.x) {
-        piv[i] = i < m ? a.piv[i] : 0;  // groups read up to kMaxDim entries
+    __shared__ int piv[kMaxM], rinfo[kMaxM];
+    for (int i = threadIdx.x; i < kMaxM; i += blockDim.x) {
+        piv[i] = i < m ? a.piv[i] : 0;  // groups read up to kMaxM entries
         rinfo[i] = i < m ? a.rowinfo[i] : 0;
     }
     __syncthreads();
@@ -816,7 +886,7 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
             const int ri = rinfo[i];
             uint64_t v;
             if (ri >= 0) {
-                v = d < L ? row_combo<SMALL>(src, a, piv, ri, rank, cap, m, j, d, own(src[(size_t)e * cap + d]))
+                v = d < L ? row_combo<SMALL, BIG>(src, a, piv, ri, rank, cap, m, j, d, own(src[(size_t)e * cap + d]))
                           : 0;
             } else {
                 v = d >= 1 ? src[(size_t)e * cap + d - 1] : 0;
@@ -831,7 +901,7 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
             const int i = (int)(e / (unsigned)n), j = (int)(e - (unsigned)i * n);
             const int ri = rinfo[i];
             const uint64_t v =
-                ri >= 0 ? row_combo<SMALL>(rs, a, piv, ri, rank, a.ordL, n, j, d, own(rs[(size_t)e * a.ordL + d]))
+                ri >= 0 ? row_combo<SMALL, BIG>(rs, a, piv, ri, rank, a.ordL, n, j, d, own(rs[(size_t)e * a.ordL + d]))
                         : rs[(size_t)e * a.ordL + d - 1];
             rd[(size_t)e * a.ordL + d] = v;
             if (d == ord + 1)
@@ -843,7 +913,7 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
 // scale[i] <- 1 / scale[i] as a plain residue (the scales are Montgomery
 // residues, nonzero as products of pivots)
 template <typename Arith>
-__global__ void __launch_bounds__(kMaxDim) leaf_scale_inv_kernel(LeafArgs a)
+__global__ void __launch_bounds__(kMaxM) leaf_scale_inv_kernel(LeafArgs a)
 {
     const Arith ar(a.mc);
     const int i = threadIdx.x;
@@ -970,24 +1040,27 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     hipLaunchKernelGGL(leaf_init_kernel, dim3(grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL)),
                        dim3(256), 0, s, a, F, strideF, lt);
     HIP_TRY(hipGetLastError(), "leaf_init_kernel");
-    // LDS of the step kernel: shifts, At (n rows of odd stride m | 1),
-    // pivot cofactors, two coefficient buffers, then 7 int arrays (sizes as in the kernel)
-    const size_t lds = ((size_t)n * (m | 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 + 7 * kMaxDim * sizeof(int);
+    const bool small = ctx->p < ((uint64_t)1 << 30);
+    const bool big = ctx->p >= ((uint64_t)1 << 62);
+    const size_t lds = step_lds_bytes(m, n, small ? 4 : 8);
+    if (lds > kStepLdsMax)
+        return fail(PMLG_ERR_UNSUPPORTED, "M-Basis: an m x n residual of this size does not fit "
+                                          "the step kernel's LDS");
     static bool attr = false;
     if (!attr) {
-        const int maxlds = (int)(((size_t)kMaxDim * (kMaxDim + 1) + (2 + 2 * (kPanel + 1)) * kMaxDim) * 8 +
-                                 7 * kMaxDim * sizeof(int));
+        const int maxlds = (int)kStepLdsMax;
         const void *kernels[] = {(const void *)mbasis_step_kernel<Mont64Arith, 1>,
                                  (const void *)mbasis_step_kernel<Mont64Arith, 2>,
                                  (const void *)mbasis_step_kernel<Mont32Arith, 1>,
-                                 (const void *)mbasis_step_kernel<Mont32Arith, 2>};
+                                 (const void *)mbasis_step_kernel<Mont32Arith, 2>,
+                                 (const void *)mbasis_step_kernel<BigArith, 1>,
+                                 (const void *)mbasis_step_kernel<BigArith, 2>};
         for (const void *k : kernels)
             HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, maxlds),
                     "hipFuncSetAttribute(mbasis_step_kernel)");
         attr = true;
     }
     const unsigned ugrid = grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL);
-    const bool small = ctx->p < ((uint64_t)1 << 30);
     // rows of the transposed residual per lane of the composing wave
     const bool one_row = n <= 64;
     for (int ord = 0; ord < ordL; ++ord) {
@@ -998,7 +1071,17 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
             else
                 hipLaunchKernelGGL((mbasis_step_kernel<Mont32Arith, 2>), dim3(1), dim3(kStepThreads), lds,
                                    s, a, ord);
-            hipLaunchKernelGGL(mbasis_update_kernel<true>, dim3(ugrid), dim3(256), 0, s, a, ord);
+            hipLaunchKernelGGL((mbasis_update_kernel<true, false>), dim3(ugrid), dim3(256), 0, s, a,
+                               ord);
+        } else if (big) {
+            if (one_row)
+                hipLaunchKernelGGL((mbasis_step_kernel<BigArith, 1>), dim3(1), dim3(kStepThreads), lds,
+                                   s, a, ord);
+            else
+                hipLaunchKernelGGL((mbasis_step_kernel<BigArith, 2>), dim3(1), dim3(kStepThreads), lds,
+                                   s, a, ord);
+            hipLaunchKernelGGL((mbasis_update_kernel<false, true>), dim3(ugrid), dim3(256), 0, s, a,
+                               ord);
         } else {
             if (one_row)
                 hipLaunchKernelGGL((mbasis_step_kernel<Mont64Arith, 1>), dim3(1), dim3(kStepThreads), lds,
@@ -1006,7 +1089,8 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
             else
                 hipLaunchKernelGGL((mbasis_step_kernel<Mont64Arith, 2>), dim3(1), dim3(kStepThreads), lds,
                                    s, a, ord);
-            hipLaunchKernelGGL(mbasis_update_kernel<false>, dim3(ugrid), dim3(256), 0, s, a, ord);
+            hipLaunchKernelGGL((mbasis_update_kernel<false, false>), dim3(ugrid), dim3(256), 0, s, a,
+                               ord);
         }
     }
     HIP_TRY(hipGetLastError(), "mbasis kernels");
@@ -1018,9 +1102,11 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     if (ordL > 0) {
         // divide the row scales out of the basis
         if (small)
-            hipLaunchKernelGGL(leaf_scale_inv_kernel<Mont32Arith>, dim3(1), dim3(kMaxDim), 0, s, a);
+            hipLaunchKernelGGL(leaf_scale_inv_kernel<Mont32Arith>, dim3(1), dim3(kMaxM), 0, s, a);
+        else if (big)
+            hipLaunchKernelGGL(leaf_scale_inv_kernel<BigArith>, dim3(1), dim3(kMaxM), 0, s, a);
         else
-            hipLaunchKernelGGL(leaf_scale_inv_kernel<Mont64Arith>, dim3(1), dim3(kMaxDim), 0, s, a);
+            hipLaunchKernelGGL(leaf_scale_inv_kernel<Mont64Arith>, dim3(1), dim3(kMaxM), 0, s, a);
         hipLaunchKernelGGL(leaf_rescale_kernel, dim3(grid_for((int64_t)m * m * L)), dim3(256), 0, s,
                            a.app[res], (int64_t)cap, m, (int64_t)L, a.scale, ctx->mc);
         HIP_TRY(hipGetLastError(), "leaf_rescale_kernel");
@@ -1195,8 +1281,12 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
                          int64_t lenF, int64_t order, int method, bool iterative, hipStream_t s,
                          uint64_t *hP)
 {
-    if (m <= 0 || n <= 0 || m > kMaxDim || n > kMaxDim || order < 0 || lenF < 0)
-        return fail(PMLG_ERR_ARG, "approximant basis: dimensions must be in [1, 128], order >= 0");
+    if (m <= 0 || n <= 0 || m > kMaxM || n > kMaxN || order < 0 || lenF < 0)
+        return fail(PMLG_ERR_ARG, "approximant basis: 1 <= m <= 256, 1 <= n <= 128, order >= 0");
+    if (step_lds_bytes((int)m, (int)n, ctx->p < ((uint64_t)1 << 30) ? 4 : 8) > kStepLdsMax)
+        return fail(PMLG_ERR_UNSUPPORTED,
+                    "approximant basis: the m x n residual exceeds one workgroup's LDS (n (m | 1) "
+                    "words plus the step's buffers: e.g. m = 256 allows n <= 62 for p >= 2^30)");
     if (!ctx->is_prime)
         return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
     if (capP < order + 1)
@@ -1366,8 +1456,8 @@ pmlg_status pmlg_pmbasis_leaf_order(pmlg_ctx ctx, int64_t m, int64_t n, int meth
     pmlg_status st = check_ctx(ctx);
     if (st != PMLG_OK)
         return st;
-    if (m <= 0 || n <= 0 || m > kMaxDim || n > kMaxDim || !order)
-        return fail(PMLG_ERR_ARG, "pmbasis leaf order: dimensions must be in [1, 128]");
+    if (m <= 0 || n <= 0 || m > kMaxM || n > kMaxN || !order)
+        return fail(PMLG_ERR_ARG, "pmbasis leaf order: 1 <= m <= 256, 1 <= n <= 128");
     if (!ctx->is_prime)
         return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1459,8 +1549,7 @@ __global__ void eval_kernel(uint64_t *out, const uint64_t *A, int64_t entries, i
          e += (int64_t)gridDim.x * blockDim.x) {
         uint64_t v = 0;
         for (int64_t d = len - 1; d >= 0; --d) {
-            v = mulmod_g(v, x, mc) + A[e * len + d];
-            v = v >= mc.p ? v - mc.p : v;
+            v = addmod_any(mulmod_g(v, x, mc), A[e * len + d], mc.p);
         }
         out[e] = v;
     }

@@ -2,6 +2,7 @@
 
     python -m pml_amd.learn collect --shapes 22000 --out timings.npz
     python -m pml_amd.learn train --data timings.npz --out dispatch_mlp.npz
+    python -m pml_amd.learn bands --data timings.npz --out pml_amd/csrc/bands.inc
 
 `collect` draws shapes of the uneven-batch workload (m, k, n uniform in
 2..64, each length log-uniform in 16..16384, the modulus a 30-bit or a 60-bit
@@ -13,6 +14,8 @@ in a real plan).
 forward, backward and Adam kernels) on log(ms) of the first shapes and
 evaluates on the held-out rest: total recorded time of the configurations the
 network picks vs those of the threshold rule, and vs the per-shape best.
+`bands` fits the banded threshold rule (the fallback and the judge) on the
+same training shapes and writes its table for the library.
 """
 import argparse
 import ctypes
@@ -188,6 +191,65 @@ def dataset(shapes, T):
     return X, Y, M
 
 
+# ---------------------------------------------------------------------------
+# banded threshold rule (the fallback and the judge of the learned choice)
+# ---------------------------------------------------------------------------
+# Bands in the style of the reference's dispatch (mul.c:57-96, bands over the
+# dimension and the length): word size (p < 2^30 or not), the shorter
+# operand's length, the largest dimension and the product length.  Each cell
+# takes the configuration of least total recorded time over the training
+# shapes that fall in it; empty cells and cells whose best configuration
+# cannot run a shape fall back to the fixed rule (direct convolution for
+# operands of at most 8 coefficients, else MFMA when allowed, else plain).
+BAND_SHORT = (8, 32, 128, 512, 2048)  # upper bounds of min(lenA, lenB) bands 0..4, then 5
+BAND_DIM = (8, 16, 32)                # upper bounds of max(m, k, n) bands 0..2, then 3
+NBANDS = 2 * 6 * 4 * 4
+
+
+def band_index(p, m, k, n, la, lb):
+    """Cell of a shape (the same function as band_index in batch.hip)."""
+    w = 0 if p < (1 << 30) else 1
+    short = min(la, lb)
+    sb = next((i for i, b in enumerate(BAND_SHORT) if short <= b), len(BAND_SHORT))
+    dim = max(m, k, n)
+    db = next((i for i, b in enumerate(BAND_DIM) if dim <= b), len(BAND_DIM))
+    lc = la + lb - 1
+    lg = max(0, (lc - 1).bit_length())  # ceil(log2(lc))
+    lb_ = min(3, max(0, (lg - 9) // 2))
+    return ((w * 6 + sb) * 4 + db) * 4 + lb_
+
+
+def fit_bands(shapes, T):
+    """Per cell, the configuration of least total recorded time over the
+    shapes of the cell for which every candidate ran (-1: none)."""
+    cells = {}
+    for i, (p, m, k, n, la, lb) in enumerate(shapes):
+        cells.setdefault(band_index(p, m, k, n, la, lb), []).append(i)
+    table = np.full(NBANDS, -1, dtype=np.int64)
+    for b, idx in cells.items():
+        best = None
+        for c in range(OUT):
+            ts = T[idx, c]
+            if (ts <= 0).any():
+                continue
+            if best is None or ts.sum() < best[0]:
+                best = (ts.sum(), c)
+        if best is not None:
+            table[b] = best[1]
+    return table
+
+
+def write_bands(table, path, ntrain, source):
+    rows = ", ".join(str(int(x)) for x in table)
+    with open(path, "w") as f:
+        f.write("// Generated by `python -m pml_amd.learn bands` from the recorded timings\n"
+                "// (%s, the first %d shapes): the configuration of each cell of the\n"
+                "// banded threshold rule (-1: the fixed rule decides).  Cell index:\n"
+                "// ((word * 6 + short_band) * 4 + dim_band) * 4 + length_band.\n"
+                % (source, ntrain))
+        f.write("static const signed char kBandConfig[%d] = {%s};\n" % (len(table), rows))
+
+
 def threshold_choice(ctx_by_p, shape):
     p, m, k, n, la, lb = shape
     return choose_config(ctx_by_p[p], m, k, n, la, lb, "threshold")
@@ -225,6 +287,10 @@ def main(argv=None):
     c.add_argument("--seed", type=int, default=2024)
     c.add_argument("--reps", type=int, default=1)
     c.add_argument("--out", required=True)
+    bd = sub.add_parser("bands")
+    bd.add_argument("--data", default=None)
+    bd.add_argument("--train", type=int, default=20000)
+    bd.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
     t.add_argument("--train", type=int, default=20000)
@@ -239,6 +305,13 @@ def main(argv=None):
         T = collect(shapes, reps=a.reps, progress=500)
         np.savez(a.out, shapes=np.array(shapes, dtype=np.uint64), times=T)
         print("collected %d shapes in %.0f s -> %s" % (len(shapes), time.time() - t0, a.out))
+        return 0
+    if a.cmd == "bands":
+        shapes, T = load_timings(a.data or DEFAULT_TIMINGS)
+        ntr = min(a.train, len(shapes))
+        table = fit_bands(shapes[:ntr], T[:ntr])
+        write_bands(table, a.out, ntr, os.path.basename(a.data or DEFAULT_TIMINGS))
+        print("wrote %s: %d of %d cells fitted" % (a.out, int((table >= 0).sum()), len(table)))
         return 0
     with np.load(a.data, allow_pickle=False) as z:
         shapes = [tuple(int(v) for v in row) for row in z["shapes"]]
@@ -263,9 +336,6 @@ def main(argv=None):
     return 0
 
 
-if __name__ == "__main__":
-    sys.exit(main())
-
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_MODEL = os.path.join(DATA_DIR, "dispatch_mlp.npz")
@@ -286,3 +356,7 @@ def install_default(device=0):
     m = Mlp.load(DEFAULT_MODEL, device=device)
     m.install()
     return m
+
+
+if __name__ == "__main__":
+    sys.exit(main())

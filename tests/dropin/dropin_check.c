@@ -22,6 +22,8 @@
 static const uint64_t P60 = 1152921504556515329ULL;
 static const uint64_t P30 = 998244353ULL;
 static const uint64_t P_NOFFT = 723501827743829161ULL; /* no 2^k roots: multi-prime path */
+static const uint64_t P63 = 9223372036854775783ULL;     /* largest prime below 2^63 */
+static const uint64_t P64 = 18446744073709551557ULL;    /* largest prime below 2^64 */
 static uint64_t g_state = 12345;
 
 static void fail(const char *what)
@@ -233,8 +235,11 @@ static void check_basis(int iterative, slong m, slong n, slong lenF, slong order
      * test checks the degree only (verification.c:73-84), accepts it */
     for (slong j = 0; j < m; j++) {
         e = nmod_poly_mat_entry(P, 0, j);
-        for (slong d = 0; d + 1 < e->length; d++)
-            e->coeffs[d] = (e->coeffs[d] + e->coeffs[d + 1]) % p;
+        for (slong d = 0; d + 1 < e->length; d++) {
+            /* residues of up to 64 bits: the sum may carry out */
+            const uint64_t a = e->coeffs[d], b = e->coeffs[d + 1], t = a + b;
+            e->coeffs[d] = (t < a || t >= p) ? t - p : t;
+        }
     }
     if (!nmod_poly_mat_is_approximant_basis(P, F, order, shift0, ROW_LOWER))
         fail("is_approximant_basis rejects (x + 1) * row 0, which the reference accepts");
@@ -449,6 +454,11 @@ int main(void)
     check_mul(nmod_poly_mat_mul_vandermonde2, "nmod_poly_mat_mul_vandermonde2", 6, 5, 4, 200, 1, 0, P30);
     check_mul(nmod_poly_mat_mul_waksman, "nmod_poly_mat_mul_waksman", 6, 5, 4, 200, 1, 0, P30);
     check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 6, 5, 7, 400, 1, 1, P_NOFFT);
+    /* 63- and 64-bit moduli (the reference's tests draw primes up to 64 bits) */
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 4, 5, 3, 60, 1, 1, P63);
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 3, 4, 5, 50, 1, 0, P64);
+    /* a product of length 199,999 (transform length 2^18) */
+    check_mul(nmod_poly_mat_multiply, "nmod_poly_mat_multiply", 2, 2, 2, 100000, 0, 0, P60);
     check_mul_alias(6, 100, P60);
     check_mulmid(7, 5, 6, 200, 300, 100, 300, 0, P60);
     check_mulmid(7, 5, 6, 200, 300, 0, 1000, 0, P60); /* nhi beyond the product length */
@@ -456,17 +466,22 @@ int main(void)
     check_mulmid(4, 4, 4, 50, 60, 80, 80, 0, P30);    /* empty range */
     check_mulmid(8, 8, 4, 65, 128, 64, 128, 1, P30);  /* PM-Basis residual shape */
     check_mulmid(3, 2, 5, 40, 400, 39, 400, 1, P60);
+    check_mulmid(3, 4, 2, 50, 70, 20, 90, 0, P64);
     check_basis(0, 6, 3, 100, 100, 3, P60);
     check_basis(0, 8, 4, 150, 150, 0, P30);
     check_basis(0, 5, 5, 64, 64, 1, P30); /* square F */
     check_basis(1, 4, 2, 40, 40, 2, P30);
     check_basis(1, 7, 3, 25, 25, 0, P60);
     check_basis(0, 5, 2, 90, 90, 1, P_NOFFT);
+    check_basis(0, 5, 3, 60, 60, 1, P64);   /* 64-bit prime: plain-residue M-Basis steps */
+    check_basis(1, 4, 2, 30, 30, 2, P63);
+    check_basis(0, 160, 80, 40, 40, 1, P60); /* more than 128 rows */
     check_kernel(6, 2, 8, 2, P60);
     check_kernel(7, 4, 5, 2, P30);
     check_kernel(5, 5, 6, 3, P30);
     check_kernel(9, 3, 40, 3, P60);
     check_kernel(4, 6, 3, 1, P_NOFFT);
+    check_kernel(6, 2, 8, 2, P64);
     check_many_moduli();
     printf("all drop-in checks passed\n");
     return 0;

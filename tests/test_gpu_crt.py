@@ -14,7 +14,13 @@ pytestmark = pytest.mark.gpu
 P_NOFFT_MUL = 723501827743829161    # 60-bit non-FFT prime of the reference's matmul benchmark
 P_NOFFT_PMB = 674830283164631843    # 60-bit non-FFT prime of the reference's PM-Basis benchmark
 P_BIG = 4611686018427387847         # largest prime below 2^62 (above every CRT prime)
-COMPOSITES = [2, 2**61, 10**18 + 9 * 7, 3 * 5 * 7 * 11 * 13 * 17 * 19 * 23 * 29 * 31 * 37 * 41 * 43 * 47]
+COMPOSITES = [2, 2**61, 10**18 + 9 * 7, 3 * 5 * 7 * 11 * 13 * 17 * 19 * 23 * 29 * 31 * 37 * 41 * 43 * 47,
+              2**63, 2**64 - 1]
+# moduli of 63 and 64 bits (the reference's tests draw primes up to 64 bits,
+# t-mulmid.c:53, t-mul_geometric.c:60): every product takes the CRT path and
+# the recombination reduces with a division by the normalised modulus
+WORD_PRIMES = [4611686018427388039, 9223372036854775783, 9223372036854775837,
+               18446744073709551557]
 SMALL_NOFFT = [3, 1000003, 2**31 - 1]
 
 
@@ -37,7 +43,8 @@ CASES = [(1, 1, 1, 1, 1), (3, 4, 2, 50, 70), (8, 8, 8, 300, 200), (2, 64, 3, 129
          (5, 1, 6, 2000, 2)]
 
 
-@pytest.mark.parametrize("p", [P_NOFFT_MUL, P_NOFFT_PMB, P_BIG] + SMALL_NOFFT + COMPOSITES)
+@pytest.mark.parametrize("p", [P_NOFFT_MUL, P_NOFFT_PMB, P_BIG] + SMALL_NOFFT + COMPOSITES +
+                         WORD_PRIMES)
 @pytest.mark.parametrize("case", CASES)
 def test_crt_product_matches_oracle(pml, p, case):
     m, k, n, la, lb = case
@@ -50,14 +57,15 @@ def test_crt_product_matches_oracle(pml, p, case):
 
 
 def test_crt_extreme_residues(pml):
-    for p in (P_NOFFT_MUL, P_BIG):
-        A = np.full((4, 64, 300), p - 1, dtype=np.uint64)
-        B = np.full((64, 3, 500), p - 1, dtype=np.uint64)
+    for p in (P_NOFFT_MUL, P_BIG) + tuple(WORD_PRIMES) + (2**64 - 1,):
+        la, lb = (300, 500) if p < 2**62 else (40, 50)
+        A = np.full((4, 64, la), p - 1, dtype=np.uint64)
+        B = np.full((64, 3, lb), p - 1, dtype=np.uint64)
         C = ctx_for(pml, p).mul(A, B)
         assert np.array_equal(C, O.polmat_mul(A, B, p, method="school"))
 
 
-@pytest.mark.parametrize("p", [P_NOFFT_MUL, P_BIG, 2**61])
+@pytest.mark.parametrize("p", [P_NOFFT_MUL, P_BIG, 2**61] + WORD_PRIMES)
 def test_crt_mulmid_matches_oracle(pml, p):
     rng_cases = [(3, 5, 2, 100, 300, 50, 250), (4, 4, 4, 64, 65, 64, 128), (2, 3, 2, 900, 40, 0, 2000)]
     for (m, k, n, la, lb, nlo, nhi) in rng_cases:
@@ -107,10 +115,34 @@ def test_composite_modulus_rejects_bases(pml):
 
 
 def test_crt_length_limit_reported(pml):
+    """Beyond the CRT primes' largest transform (2^24 points)."""
     ctx = ctx_for(pml, P_NOFFT_MUL)
-    A = np.zeros((1, 1, 20000), dtype=np.uint64)
-    B = np.zeros((1, 1, 20000), dtype=np.uint64)
-    A[0, 0, -1] = B[0, 0, -1] = 1
+    A = np.zeros((1, 1, (1 << 23) + 1), dtype=np.uint64)
+    A[0, 0, -1] = 1
     with pytest.raises(pml.PmlgError) as ei:
-        ctx.mul(A, B)
+        ctx.mul(A, A)
     assert ei.value.status == 2  # PMLG_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("p", WORD_PRIMES)
+def test_bases_modulo_63_and_64_bit_primes(pml, p):
+    """Approximant and kernel bases for primes of 62 to 64 bits: products by
+    the CRT path, M-Basis steps in plain residues with exact 128-by-64
+    reductions (the Montgomery forms' lazy sums need p < 2^62)."""
+    from oracle import kernel as KO
+    rng = np.random.default_rng(p % 1013)
+    ctx = ctx_for(pml, p)
+    for (m, n, order, iterative) in [(6, 3, 90, False), (5, 2, 40, True), (9, 4, 70, False)]:
+        F = O.rand_uniform((m, n, order), p, m * 7 + order)
+        sh = rng.integers(0, 4, m).astype(np.int64)
+        P_ref, s_ref = (O.mbasis if iterative else O.pmbasis)(F, order, sh, p)
+        P_dev, s_dev = ctx.mbasis(F, order, sh) if iterative else ctx.pmbasis(F, order, sh)
+        assert np.array_equal(O.normalise(P_dev), O.normalise(P_ref)), (p, m, n, order)
+        assert np.array_equal(s_dev, s_ref)
+        assert O.is_approximant_basis(O.normalise(P_dev), F, order, sh, p) == 0
+    F = O.rand_uniform((6, 2, 9), p, 3)
+    for algo, fn in (("zls", KO.kernel_zls), ("via_approx", KO.kernel_via_approx)):
+        nul_r, K_r, piv_r, sh_r = fn(F, np.zeros(6, dtype=np.int64), p)
+        nul, K, piv, sh = ctx.kernel_basis(F, np.zeros(6, dtype=np.int64), algo)
+        assert nul == nul_r == 4
+        assert np.array_equal(O.normalise(K), O.normalise(K_r)) and np.array_equal(piv, piv_r)

@@ -185,8 +185,8 @@ def test_kernel_capacity_protocol(pml):
 
 def test_kernel_rejects_bad_arguments(pml):
     ctx = ctx_for(pml, P30)
-    with pytest.raises(pml.PmlgError):
-        ctx.kernel_basis(np.zeros((129, 1, 2), dtype=np.uint64), np.zeros(129, dtype=np.int64))
+    with pytest.raises(pml.PmlgError):  # more than 256 rows
+        ctx.kernel_basis(np.zeros((257, 1, 2), dtype=np.uint64), np.zeros(257, dtype=np.int64))
     with pytest.raises(pml.PmlgError):
         ctx.kernel_basis(np.ones((2, 130, 1), dtype=np.uint64), np.zeros(2, dtype=np.int64),
                          "via_approx")
@@ -208,3 +208,12 @@ def test_kernel_many_columns(pml, m, n, ln, p, algo, kind):
     rng = np.random.default_rng(m * 131 + n)
     F = family(rng, kind, m, n, ln, p)
     check(pml, p, F, np.zeros(m, dtype=np.int64), algo)
+
+
+def test_kernel_basis_with_more_than_128_rows(pml):
+    """150 and 256 rows (the approximant bases inside take up to 256 rows)."""
+    for (m, n, d, p, algo) in [(150, 40, 3, P60, "zls"), (150, 40, 3, P60, "via_approx"),
+                               (256, 20, 2, P30, "zls")]:
+        F = O.rand_uniform((m, n, d + 1), p, m + n + d)
+        nul, K = check(pml, p, F, np.zeros(m, dtype=np.int64), algo)
+        assert nul == m - n

@@ -104,7 +104,7 @@ def test_learned_dispatch_picks_argmin_and_stays_exact(learn):
             assert np.array_equal(C, O.polmat_mul(A, B, ctx.p))
         # the threshold rule is still available beside the model
         for (p, m, k, n, la, lb) in shapes:
-            assert pml_amd.choose_config(ctxs[p], m, k, n, la, lb, "threshold") in (0, 1, 2)
+            assert pml_amd.choose_config(ctxs[p], m, k, n, la, lb, "threshold") in range(8)
     finally:
         pml_amd.set_refine(True)
         learn.Mlp.uninstall()
@@ -159,13 +159,46 @@ def test_learned_dispatch_gains_over_threshold_on_unseen_shapes(learn):
         print("held-out: learned %.1f ms, threshold %.1f ms, best %.1f ms, gain %.3f" % (
             ev["learned"], ev["threshold"], ev["best"], ev["gain_vs_threshold"]))
         assert ev["shapes"] == 2000
-        assert ev["gain_vs_threshold"] > 1.04
+        # the judge is the banded rule fitted on the same 20,000 shapes
+        # (1.03 measured; 1.10 against the earlier fixed three-line rule)
+        assert ev["gain_vs_threshold"] > 1.015
         assert ev["learned"] <= 1.03 * ev["best"]
         mlp.close()
         packaged = learn.Mlp.load(learn.DEFAULT_MODEL)
         ev2 = learn.evaluate(packaged, shapes[20000:], T[20000:], ctx_by_p)
-        assert ev2["gain_vs_threshold"] > 1.04
+        assert ev2["gain_vs_threshold"] > 1.015
         packaged.close()
     finally:
         for c in ctx_by_p.values():
             c.close()
+
+
+def test_banded_threshold_rule_matches_its_table(learn):
+    """The library's threshold rule is the banded table (bands.inc, fitted by
+    learn.fit_bands on the first 20,000 recorded shapes) with the fixed rule
+    as its fallback: the library's choice equals the table lookup done here
+    for 600 shapes of the workload, and refitting reproduces the table."""
+    import os
+    import re
+    import pml_amd
+    shapes, T = learn.load_timings()
+    table = learn.fit_bands(shapes[:20000], T[:20000])
+    src = open(os.path.join(os.path.dirname(pml_amd.__file__), "csrc", "bands.inc")).read()
+    shipped = [int(x) for x in re.search(r"\{([^}]*)\}", src).group(1).split(",")]
+    assert shipped == [int(x) for x in table]
+    ctxs = {P60: pml_amd.Context(P60), P30: pml_amd.Context(P30)}
+    try:
+        for (p, m, k, n, la, lb) in learn.draw_shapes(600, 99):
+            ctx = ctxs[p]
+            c = int(table[learn.band_index(p, m, k, n, la, lb)])
+            if not (c >= 0 and pml_amd.config_valid(ctx, m, k, n, la, lb, c)):
+                if min(la, lb) <= 8 and pml_amd.config_valid(ctx, m, k, n, la, lb, 0):
+                    c = 0
+                elif pml_amd.config_valid(ctx, m, k, n, la, lb, 2):
+                    c = 2
+                else:
+                    c = 1
+            assert pml_amd.choose_config(ctx, m, k, n, la, lb, "threshold") == c, (p, m, k, n, la, lb)
+    finally:
+        for x in ctxs.values():
+            x.close()

@@ -102,12 +102,16 @@ def test_mul_zero_and_empty(pml):
     assert ctx.mul(np.zeros((3, 0, 3), np.uint64), np.zeros((0, 2, 3), np.uint64)).shape == (3, 2, 5)
 
 
-def test_unsupported_length_raises(pml):
-    # 786433 - 1 = 3 * 2^18: a product of length 2^19 needs a root the field lacks
-    ctx = ctx_for(pml, 786433)
-    A = np.ones((1, 1, 300000), dtype=np.uint64)
-    with pytest.raises(pml.PmlgError):
-        ctx.mul(A, A)
+def test_length_beyond_the_roots_takes_the_multiprime_path(pml):
+    # 786433 - 1 = 3 * 2^18: a product of length 2^19 needs a root the field
+    # lacks, so it is computed modulo the three CRT primes
+    p = 786433
+    ctx = ctx_for(pml, p)
+    A = O.rand_uniform((1, 1, 300000), p, 4)
+    C = ctx.mul(A, A)
+    for r in (5, 777):
+        a = int(O.polmat_eval(A, r, p)[0, 0])
+        assert int(O.polmat_eval(C, r, p)[0, 0]) == a * a % p
 
 
 MULMID_CASES = [

@@ -240,3 +240,44 @@ def test_pmbasis_measured_leaf_order(pml):
     for bad in (-1, 257):
         with pytest.raises(pml.PmlgError):
             ctx.set_leaf_order(bad)
+
+
+# Bases of more than 128 rows (up to 256; at most 128 columns, within one
+# workgroup's LDS: n (m | 1) words for the transposed residual).  The
+# reference has no such cap (approximant_basis.c:33-69,
+# nmod_mat_poly_mbasis.c:116-378).
+WIDE = [
+    # m, n, order, p, iterative
+    (160, 80, 40, P60, False),
+    (200, 50, 48, P30, False),
+    (256, 62, 24, P60, False),   # the largest n for m = 256 with 64-bit words
+    (256, 100, 20, P30, False),  # 32-bit words: n up to 128 at m = 256
+    (150, 60, 20, P60, True),
+    (129, 128, 36, P30, False),
+]
+
+
+@pytest.mark.parametrize("case", WIDE)
+def test_bases_with_more_than_128_rows(pml, case):
+    m, n, order, p, iterative = case
+    F = O.rand_uniform((m, n, order), p, m + n)
+    shift = np.random.default_rng(m * n).integers(0, 4, size=m).astype(np.int64)
+    ref_fn = O.mbasis if iterative else O.pmbasis
+    P_ref, s_ref = ref_fn(F, order, shift, p)
+    ctx = ctx_for(pml, p)
+    P_dev, s_dev = (ctx.mbasis(F, order, shift) if iterative else ctx.pmbasis(F, order, shift))
+    assert np.array_equal(O.normalise(P_dev), O.normalise(P_ref)), case
+    assert np.array_equal(s_dev, s_ref)
+    assert O.is_approximant_basis(O.normalise(P_dev), F, order, shift, p) == 0
+    print("basis %s identical" % (case,), flush=True)
+
+
+def test_wide_basis_beyond_lds_refused(pml):
+    ctx = ctx_for(pml, P60)
+    F = O.rand_uniform((256, 64, 8), P60, 1)
+    with pytest.raises(pml.PmlgError) as ei:
+        ctx.pmbasis(F, 8, np.zeros(256, dtype=np.int64))
+    assert ei.value.status == 2 and "LDS" in str(ei.value)
+    with pytest.raises(pml.PmlgError) as ei:
+        ctx.pmbasis(O.rand_uniform((257, 4, 8), P60, 1), 8, np.zeros(257, dtype=np.int64))
+    assert ei.value.status == 1
