@@ -13,9 +13,13 @@ inputs and outputs stay resident in HBM.  `value` is products per second of
 that batched+learned form.  Beside it: the plain GPU form (one product per
 call, plain per-lane pointwise kernel), the single-call MFMA form, the batched
 form under the threshold rule, and (when the refinement changed the pick) the
-batched form with the network's own pick.  Prints ONE JSON line.  The CPU
-baseline is the oracle's own NTT product (a single-threaded C port of the
-same algorithm) timed on this host on a bounded sample.
+batched form with the network's own pick.  Two more workloads are timed in
+the same run: the PM-Basis of the 64x32 order-2048 matrix of the golden
+fixture (its digest checked) and the 512-product uneven batch under the
+learned dispatch and the threshold rule.  Prints ONE JSON line.  The CPU
+baseline is the oracle's own NTT product (a C port of the same algorithm),
+built for this host with -O3 -march=native and timed on one thread and on
+every core this process may use, on a bounded sample.
 """
 import argparse
 import json
@@ -35,6 +39,7 @@ LENC = 2 * LEN - 1
 NPTS = 8192
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.json")
+GOLDEN_FILE = os.path.join(ROOT, "tests", "golden", "golden.json")
 
 
 def parse():
@@ -47,6 +52,8 @@ def parse():
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the comparison forms")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the PM-Basis (64x32, order 2048) and 512-product batch legs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (reference "
                          "kernel for calibrating hardware byte counters)")
@@ -65,15 +72,110 @@ def time_single(ctx, dC, dA, dB, method, steps, warmup):
     return steps / (time.perf_counter() - t0)
 
 
-def time_plan(plan, steps, warmup):
+def time_plan(plan, steps, warmup, staged=False):
     for _ in range(warmup):
         plan.run()
     plan.sync()
+    if staged:
+        plan.stage_times()  # the stage counts restart with the timed region
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.run()
     plan.sync()
     return time.perf_counter() - t0
+
+
+def splitmix_uniform(n, p, seed):
+    """Residues in [0, p) from the SplitMix64 stream of `seed` (raw draw t is
+    mix(seed + t * 0x9e3779b97f4a7c15), t = 1, 2, ...), each draw's top
+    bitlen(p - 1) bits kept when below p: the stream the golden fixtures were
+    written from, so the bench runs the fixture's own inputs."""
+    bits = int(p - 1).bit_length()
+    out, got, t = [], 0, 0
+    with np.errstate(over="ignore"):
+        while got < n:
+            chunk = int((n - got) * 1.1) + 64
+            z = np.uint64(seed) + np.arange(t + 1, t + chunk + 1, dtype=np.uint64) * \
+                np.uint64(0x9e3779b97f4a7c15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+            z = z ^ (z >> np.uint64(31))
+            x = z >> np.uint64(64 - bits)
+            x = x[x < np.uint64(p)][:n - got]
+            out.append(x)
+            got += len(x)
+            t += chunk
+    return np.concatenate(out)
+
+
+def bench_c3(reps):
+    """PM-Basis of the 64x32 order-2048 matrix of the golden fixture (30-bit
+    prime, zero shift), F and P resident in HBM; the basis is compared with
+    the fixture's digest."""
+    import hashlib
+    import pml_amd
+    g = json.load(open(GOLDEN_FILE))["c3_pmbasis_64x32_order2048_p30"]
+    m, n, order, p = g["m"], g["n"], g["order"], g["p"]
+    F = splitmix_uniform(m * n * order, p, g["seedF"]).reshape(m, n, order)
+    cap = order + 1
+    with pml_amd.Context(p) as ctx:
+        dF = pml_amd.DeviceArray(ctx, F.nbytes)
+        dF.upload(F)
+        dP = pml_amd.DeviceArray(ctx, m * m * cap * 8)
+        sh = np.zeros(m, dtype=np.int64)
+        L = ctx.pmbasis_dev(dP.ptr, cap, sh, dF.ptr, m, n, order, order)  # leaf order, arena
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sh = np.zeros(m, dtype=np.int64)
+            L = ctx.pmbasis_dev(dP.ptr, cap, sh, dF.ptr, m, n, order, order)
+        dt = (time.perf_counter() - t0) / reps
+        P = dP.download(np.empty((m, m, cap), dtype=np.uint64))[:, :, :L]
+        while P.shape[2] and not P[:, :, -1].any():
+            P = P[:, :, :-1]
+        ok = hashlib.sha256(np.ascontiguousarray(P).tobytes()).hexdigest() == g["sha256"] and \
+            [int(x) for x in sh] == g["shift_out"]
+        leaf = ctx.leaf_order(m, n)
+        dF.free()
+        dP.free()
+    return {"seconds_per_basis": round(dt, 5), "bases_per_s": round(1.0 / dt, 2),
+            "reps": reps, "leaf_order": leaf, "golden_digest_match": bool(ok),
+            "workload": "pmbasis 64x32, order 2048, p=998244353, zero shift, F and P in HBM"}
+
+
+def bench_c4(reps):
+    """512 uneven products (m, k, n uniform in 2..64, lengths log-uniform in
+    16..16384, 30-/60-bit FFT primes 50/50, seed 4), operands generated on
+    the device, one batch plan replayed as a graph: learned dispatch (with the
+    measured refinement) and the threshold rule."""
+    import ctypes
+    from pml_amd import learn, lib, _check
+    from pml_amd.perf import DeviceBatch
+    shapes = learn.draw_shapes(512, 4)
+    batch = DeviceBatch(shapes, "auto")
+    out = {"workload": "512 uneven products, m,k,n in 2..64, lengths 16..16384 log-uniform, "
+                       "p in {998244353, 2^60-3*2^24+1}, seed 4",
+           "algorithmic_bytes_per_batch": batch.bytes, "reps": reps}
+    for name, meth in (("learned", 0), ("threshold", 3)):
+        for i in range(len(shapes)):
+            batch.descs[i].method = meth
+        t0 = time.perf_counter()
+        plan = batch.plan(1)
+        plan_s = time.perf_counter() - t0
+        _check(lib().pmlg_batch_plan_run(plan, None))  # eager pass + capture
+        _check(lib().pmlg_batch_plan_run(plan, None))
+        _check(lib().pmlg_batch_plan_sync(plan))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _check(lib().pmlg_batch_plan_run(plan, None))
+        _check(lib().pmlg_batch_plan_sync(plan))
+        dt = (time.perf_counter() - t0) / reps
+        launches, arena = ctypes.c_int64(), ctypes.c_uint64()
+        _check(lib().pmlg_batch_plan_stats(plan, ctypes.byref(launches), ctypes.byref(arena)))
+        lib().pmlg_batch_plan_destroy(plan)
+        out[name] = {"seconds_per_batch": round(dt, 5), "products_per_s": round(512 / dt, 1),
+                     "plan_seconds": round(plan_s, 2), "launches": launches.value}
+    batch.close()
+    return out
 
 
 def freivalds(A, B, C, p, seed):
@@ -91,18 +193,32 @@ def freivalds(A, B, C, p, seed):
 
 
 def cpu_baseline(A, B, budget_s):
-    """Oracle NTT product (single thread) on whole 32x32x32 products until the
-    time budget is used (at least one)."""
+    """The oracle's NTT product (a C port of the same algorithm: radix-2
+    transforms with Shoup butterflies, exact 128-bit pointwise sums), built
+    here with -O3 -march=native, on whole 32x32x32 products: one thread for
+    half the budget, then every core this process may use for the other
+    half (at least one product each)."""
+    import tempfile
     from oracle import oracle as O
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        O.polmat_mul(A, B, P60, method="ntt")
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 8:
-            break
-    return n / el, n, el
+    with tempfile.TemporaryDirectory() as d:
+        flags = O.use_native(d) or "-O3 -march=x86-64-v3 -fopenmp (shipped build)"
+        cores = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+        res = {}
+        for th in (1, cores):
+            O.set_threads(th)
+            t0 = time.perf_counter()
+            n = 0
+            while True:
+                O.polmat_mul(A, B, P60, method="ntt")
+                n += 1
+                el = time.perf_counter() - t0
+                if el >= budget_s / 2 or n >= 64:
+                    break
+            res[th] = (n / el, n, el)
+        O.set_threads(0)
+    return res, cores, flags
 
 
 def stage_bytes(batch):
@@ -142,8 +258,12 @@ def main():
     chosen = pml_amd.choose_config(ctx, M, K, N_, LEN, LEN, "auto")
 
     plan = pml_amd.BatchPlan([(ctx, As[i], Bs[i]) for i in range(Bn)], method="auto", graph=True)
-    el = time_plan(plan, args.steps, args.warmup)
+    # graph instances with events between the stages, one per timed step: the
+    # stage times below come from the timed replays themselves
+    plan.time_stages(min(args.steps, 256))
+    el = time_plan(plan, args.steps, args.warmup, staged=True)
     value = args.steps * Bn / el
+    ms, timed_replays = plan.stage_times()
 
     forms = {"batched_learned": value}
     dA = pml_amd.DeviceArray(ctx, As[0].nbytes)
@@ -171,9 +291,9 @@ def main():
         src.free()
         dst.free()
 
-    # dominant kernel: stage times of eager runs of the same plan (HIP events
-    # on the plan's stream); every stage of this batch is one launch
-    ms = plan.profile(reps=5)
+    # dominant kernel: stage times of the timed graph replays (HIP events
+    # recorded by the graph on the stream the kernels run on); every stage of
+    # this batch is one launch
     names = ["forward", "pointwise", "inverse"]
     dom = max(range(3), key=lambda i: ms[i])
     algo = stage_bytes(Bn)
@@ -200,7 +320,9 @@ def main():
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                 "traffic": traffic, "avg_launch_ms": round(ms[dom], 5),
                 "algorithmic_bytes_per_launch": algo[names[dom]],
-                "stage_ms_per_launch": {names[i]: round(ms[i], 5) for i in range(3)}}
+                "stage_ms_per_launch": {names[i]: round(ms[i], 5) for i in range(3)},
+                "stage_ms_source": "HIP events inside the timed graph replays (%d replays)"
+                                   % timed_replays}
     # the pointwise stage against the int8 MFMA peak (8x8 limb products per mod-MAC)
     int8_ops = Bn * NPTS * M * K * N_ * 64 * 2
     roofline["pointwise_int8_tops"] = round(int8_ops / (ms[1] * 1e-3) / 1e12, 1)
@@ -216,13 +338,22 @@ def main():
     plan.close()
     model.close()
 
+    extra = {}
+    if not args.no_extra:
+        extra["c3_pmbasis"] = bench_c3(3)
+        extra["c4_batch512"] = bench_c4(3)
+
     cpu = None
     if not args.no_cpu:
-        rate, n, el_cpu = cpu_baseline(As[0], Bs[0], args.cpu_seconds)
-        cpu = {"value": round(rate, 5), "unit": "products/s", "cores": 1, "kind": "port",
-               "sample": "%d whole 32x32x32 len-4096 product(s) by the oracle's C NTT path "
-                         "(Shoup butterflies, 128-bit accumulation), %.1f s on one host core"
-                         % (n, el_cpu)}
+        res, cores, flags = cpu_baseline(As[0], Bs[0], args.cpu_seconds)
+        r1, rc = res[1], res[cores]
+        cpu = {"value": round(rc[0], 5), "unit": "products/s", "cores": cores, "kind": "port",
+               "sample": "whole 32x32x32 len-4096 products by the oracle's C NTT path (Shoup "
+                         "butterflies, exact 128-bit pointwise sums) built %s: %d in %.1f s on "
+                         "%d threads, %d in %.1f s on 1 thread" % (flags, rc[1], rc[2], cores,
+                                                                   r1[1], r1[2]),
+               "value_1_thread": round(r1[0], 5), "host_cores_online": os.cpu_count(),
+               "cores_available_to_process": len(os.sched_getaffinity(0))}
 
     out = {
         "metric": "32x32 deg-4096 mod-p polmat mul/s, batched+learned vs plain GPU, bit-exact",
@@ -247,6 +378,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

@@ -31,52 +31,73 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(path)
-        u64, i64, c_int = ctypes.c_uint64, ctypes.c_int64, ctypes.c_int
-        L.orc_mulmod.restype = u64
-        L.orc_mulmod.argtypes = [u64, u64, u64]
-        L.orc_powmod.restype = u64
-        L.orc_powmod.argtypes = [u64, u64, u64]
-        L.orc_invmod.restype = u64
-        L.orc_invmod.argtypes = [u64, u64]
-        L.orc_is_prime.restype = c_int
-        L.orc_is_prime.argtypes = [u64]
-        L.orc_root_of_unity_2k.restype = u64
-        L.orc_root_of_unity_2k.argtypes = [u64, c_int]
-        L.orc_rand_uniform.restype = None
-        L.orc_rand_uniform.argtypes = [_u64p, i64, u64, u64]
-        for name in ("orc_polmat_mul_school", "orc_polmat_mul"):
-            f = getattr(L, name)
-            f.restype = None
-            f.argtypes = [_u64p, _u64p, _u64p, i64, i64, i64, i64, i64, u64]
-        L.orc_polmat_mul_ntt.restype = c_int
-        L.orc_polmat_mul_ntt.argtypes = [_u64p, _u64p, _u64p, i64, i64, i64, i64, i64, u64]
-        L.orc_polmat_mulmid.restype = None
-        L.orc_polmat_mulmid.argtypes = [_u64p, _u64p, _u64p, i64, i64, i64, i64, i64,
-                                        i64, i64, u64]
-        L.orc_polmat_eval.restype = None
-        L.orc_polmat_eval.argtypes = [_u64p, _u64p, i64, i64, i64, u64, u64]
-        L.orc_rref.restype = i64
-        L.orc_rref.argtypes = [_u64p, i64, i64, u64, _i64p]
-        L.orc_rank.restype = i64
-        L.orc_rank.argtypes = [_u64p, i64, i64, u64]
-        L.orc_left_nullspace_compact.restype = i64
-        L.orc_left_nullspace_compact.argtypes = [_u64p, _i64p, _u64p, i64, i64, u64]
-        for name in ("orc_mbasis", "orc_pmbasis"):
-            f = getattr(L, name)
-            f.restype = None
-            f.argtypes = [ctypes.POINTER(_u64p), _i64p, _i64p, _u64p, i64, i64, i64, i64, u64]
-        L.orc_is_approximant_basis.restype = c_int
-        L.orc_is_approximant_basis.argtypes = [_u64p, i64, _u64p, i64, i64, i64, i64,
-                                               _i64p, u64, u64]
-        L.orc_free.restype = None
-        L.orc_free.argtypes = [ctypes.c_void_p]
-        L.orc_set_threads.restype = None
-        L.orc_set_threads.argtypes = [c_int]
-        L.orc_threads.restype = c_int
-        L.orc_threads.argtypes = []
-        _LIB = L
+        _LIB = _load(path)
     return _LIB
+
+
+def use_native(workdir):
+    """Compile the oracle with -O3 -march=native for this host into `workdir`
+    and use that build from now on (the CPU baseline's timing leg).  Returns
+    the flags used, or None (the shipped x86-64-v3 build stays) when the
+    compiler is missing or fails."""
+    global _LIB
+    out = os.path.join(workdir, "liboracle_native.so")
+    flags = ["-O3", "-march=native", "-fopenmp", "-fPIC", "-std=gnu11", "-shared"]
+    try:
+        subprocess.run(["gcc"] + flags + [os.path.join(_HERE, "pml_oracle.c"), "-o", out],
+                       check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    except (OSError, subprocess.CalledProcessError):
+        return None
+    _LIB = _load(out)
+    return " ".join(flags[:3])
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    u64, i64, c_int = ctypes.c_uint64, ctypes.c_int64, ctypes.c_int
+    L.orc_mulmod.restype = u64
+    L.orc_mulmod.argtypes = [u64, u64, u64]
+    L.orc_powmod.restype = u64
+    L.orc_powmod.argtypes = [u64, u64, u64]
+    L.orc_invmod.restype = u64
+    L.orc_invmod.argtypes = [u64, u64]
+    L.orc_is_prime.restype = c_int
+    L.orc_is_prime.argtypes = [u64]
+    L.orc_root_of_unity_2k.restype = u64
+    L.orc_root_of_unity_2k.argtypes = [u64, c_int]
+    L.orc_rand_uniform.restype = None
+    L.orc_rand_uniform.argtypes = [_u64p, i64, u64, u64]
+    for name in ("orc_polmat_mul_school", "orc_polmat_mul"):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [_u64p, _u64p, _u64p, i64, i64, i64, i64, i64, u64]
+    L.orc_polmat_mul_ntt.restype = c_int
+    L.orc_polmat_mul_ntt.argtypes = [_u64p, _u64p, _u64p, i64, i64, i64, i64, i64, u64]
+    L.orc_polmat_mulmid.restype = None
+    L.orc_polmat_mulmid.argtypes = [_u64p, _u64p, _u64p, i64, i64, i64, i64, i64,
+                                    i64, i64, u64]
+    L.orc_polmat_eval.restype = None
+    L.orc_polmat_eval.argtypes = [_u64p, _u64p, i64, i64, i64, u64, u64]
+    L.orc_rref.restype = i64
+    L.orc_rref.argtypes = [_u64p, i64, i64, u64, _i64p]
+    L.orc_rank.restype = i64
+    L.orc_rank.argtypes = [_u64p, i64, i64, u64]
+    L.orc_left_nullspace_compact.restype = i64
+    L.orc_left_nullspace_compact.argtypes = [_u64p, _i64p, _u64p, i64, i64, u64]
+    for name in ("orc_mbasis", "orc_pmbasis"):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [ctypes.POINTER(_u64p), _i64p, _i64p, _u64p, i64, i64, i64, i64, u64]
+    L.orc_is_approximant_basis.restype = c_int
+    L.orc_is_approximant_basis.argtypes = [_u64p, i64, _u64p, i64, i64, i64, i64,
+                                           _i64p, u64, u64]
+    L.orc_free.restype = None
+    L.orc_free.argtypes = [ctypes.c_void_p]
+    L.orc_set_threads.restype = None
+    L.orc_set_threads.argtypes = [c_int]
+    L.orc_threads.restype = c_int
+    L.orc_threads.argtypes = []
+    return L
 
 
 def _p(a):
