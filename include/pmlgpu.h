@@ -30,8 +30,8 @@
  * 2^ceil(log2(N)) | p-1 for the transform length N a product needs (N <= 2^24
  * for p >= 2^30, 2^26 for p < 2^30), the product is evaluated modulo p
  * itself; otherwise (non-FFT primes, composite moduli, p >= 2^62) it is
- * computed modulo three 61-bit FFT primes and recombined by CRT (N <= 2^24,
- * k min(lenA, lenB) < 2^52).  Approximant and kernel bases need an odd prime.
+ * computed modulo three 60-bit FFT primes and recombined by CRT (N <= 2^24,
+ * k min(lenA, lenB) < 2^51).  Approximant and kernel bases need an odd prime.
  * Longer products return PMLG_ERR_UNSUPPORTED.
  */
 #ifndef PMLGPU_H

@@ -1,10 +1,11 @@
 // Products modulo any p in [2, 2^64) whose roots of unity do not reach the
 // transform length (non-FFT primes, composite moduli, every p >= 2^62): the
 // integer product of
-// the residues is computed modulo three FFT primes q1 > q2 > q3 in (2^60, 2^61)
-// with the ordinary device path, recombined by Garner's mixed-radix CRT and
+// the residues is computed modulo three FFT primes q1 > q2 > q3 in (2^59, 2^60)
+// (below 2^60, so that their transforms run in the lazy butterfly mode) with
+// the ordinary device path, recombined by Garner's mixed-radix CRT and
 // reduced mod p.  A coefficient of A*B is at most k min(lenA, lenB) (p-1)^2 <
-// 2^128 k min(lenA, lenB), below q1 q2 q3 > 2^180 for k min(lenA, lenB) < 2^52,
+// 2^128 k min(lenA, lenB), below q1 q2 q3 > 2^179.99 for k min(lenA, lenB) < 2^51,
 // so the recombined integer is the exact coefficient.  This is the
 // multi-prime scheme of ntl-extras (mat_lzz_pX_multiply_3_primes.cpp:47-200),
 // here on the GPU transforms.
@@ -17,9 +18,10 @@ namespace pmlg {
 
 namespace {
 
-constexpr uint64_t kCrtPrimes[3] = {2305842979148922881ull,   // 0x1ffffff900000001, 2^32 | q-1
-                                    2305842949084151809ull,   // 0x1ffffff200000001, 2^33 | q-1
-                                    2305842811645198337ull};  // 0x1fffffd200000001, 2^33 | q-1
+// the three largest primes c 2^32 + 1 below 2^60
+constexpr uint64_t kCrtPrimes[3] = {1152921092289986561ull,   // 0x0fffffa000000001, 2^37 | q-1
+                                    1152920989210771457ull,   // 0x0fffff8800000001, 2^35 | q-1
+                                    1152920933376196609ull};  // 0x0fffff7b00000001, 2^32 | q-1
 
 struct CrtConst {
     ModConst m2, m3, mp;     // moduli q2, q3 and p
@@ -63,7 +65,7 @@ crt_kernel(uint64_t *__restrict__ out, int64_t out_stride, const uint64_t *__res
         // x = v1 + q1 v2 + q1 q2 v3, reduced mod p
         uint64_t x;
         if (k.big) {
-            // p >= 2^62: v1, v2, v3 < 2^61 are already below p; sums may carry
+            // p >= 2^62: v1, v2, v3 < 2^60 are already below p; sums may carry
             x = addmod_any(v1, mulmod_any(k.q1_p, v2, k.mp), k.p);
             x = addmod_any(x, mulmod_any(k.q12_p, v3, k.mp), k.p);
         } else {
@@ -112,9 +114,9 @@ pmlg_status crt_product(pmlg_ctx ctx, uint64_t *dOut, int64_t out_stride, const 
             if (st != PMLG_OK)
                 return st;
         }
-    if ((unsigned __int128)k * (uint64_t)(la < lb ? la : lb) >= ((unsigned __int128)1 << 52))
+    if ((unsigned __int128)k * (uint64_t)(la < lb ? la : lb) >= ((unsigned __int128)1 << 51))
         return fail(PMLG_ERR_UNSUPPORTED, "multi-prime path: k * min(lenA, lenB) must stay below "
-                                          "2^52 for the three-prime recombination to be exact");
+                                          "2^51 for the three-prime recombination to be exact");
     const int64_t lc = la + lb - 1;
     const int64_t need = hi > lc - nlo ? hi : lc - nlo;
     if (ceil_log2(need) > ctx->crt[0]->max_log)

@@ -224,7 +224,9 @@ __device__ __forceinline__ void merge_one(const W *__restrict__ z, Wout *__restr
     const int64_t o = lo + (idx - e * width);
     const int64_t i1 = o & (M - 1);
     const W *ze = z + e * N;
-    W acc = reduce_2p<W>(ze[i1], p);
+    // z holds the row inverses' lazy words (any value below 2^bits(W)): the
+    // first term too goes through a Shoup product (by w^0 = 1), into [0, 2p)
+    W acc = tw_mul<W>(ze[i1], tab.itw, tab.itwq, 0, p);
     for (int g = 1; g < R; ++g) {
         const int64_t widx = (o * g) & (N - 1);
         acc = add_2p<W>(acc, tw_mul<W>(ze[(int64_t)g * M + i1], tab.itw, tab.itwq, widx, p), p2);
@@ -282,15 +284,54 @@ __device__ __forceinline__ void load_tw(const W *__restrict__ tab, int idx, W &w
     }
 }
 
-// Butterfly constants.  Strict mode keeps values in [0, 2p) (any p < 2^62);
-// loose mode (64-bit words, p < 2^61) keeps them in [0, 4p) in the forward
-// transform and [0, 8p) in the inverse, and uses a cheaper Shoup product
-// whose quotient may be short by at most 2.
+// Butterfly constants.  Strict mode keeps values in [0, 2p) (any p < 2^62).
+// Lazy mode (64-bit words, p < 2^60, so that 16p < 2^64) lets values grow up
+// to 16p and tracks a bound for every register at compile time (the
+// fwd_bound / inv_bound rules below): the Shoup product returns [0, 4p) for
+// any 64-bit input, so only sums grow, and a conditional subtraction of 8p is
+// placed only where a bound would pass 16p.
 template <typename W>
 struct Bfly {
-    W p, p2, p4, np, np4;  // np = -p mod 2^64, np4 = -4p mod 2^64
-    __device__ explicit Bfly(W pp) : p(pp), p2(pp + pp), p4(4 * pp), np((W)0 - pp), np4((W)0 - 4 * pp) {}
+    W p, p2, p4, p8;
+    W np, np2, np4, np8;  // -p, -2p, -4p, -8p mod 2^bits(W)
+    __device__ explicit Bfly(W pp)
+        : p(pp), p2(pp + pp), p4(4 * pp), p8(8 * pp), np((W)0 - pp), np2((W)0 - 2 * pp),
+          np4((W)0 - 4 * pp), np8((W)0 - 8 * pp)
+    {
+        // keep the negatives opaque: knowing np4 = -p4, the compiler turns
+        // the select of s or s + np4 into a subtraction with a borrow chain
+        // (VCC carry plus a hazard wait); p is uniform, so SGPRs are exact
+        asm volatile("" : "+s"(np), "+s"(np2), "+s"(np4), "+s"(np8));
+    }
 };
+
+// x >= c ? x - c : x, with nc = -c: the candidate is formed by one add and
+// selected, so no borrow chain feeds the select
+template <typename W>
+__device__ __forceinline__ W csub(W x, W c, W nc)
+{
+    const W t = x + nc;
+    return x >= c ? t : x;
+}
+
+// B p for B in {1, 2, 4, 8}
+template <int B, typename W>
+__device__ __forceinline__ W mult_p(const Bfly<W> &k)
+{
+    static_assert(B == 1 || B == 2 || B == 4 || B == 8, "multiple of p");
+    return B == 1 ? k.p : B == 2 ? k.p2 : B == 4 ? k.p4 : k.p8;
+}
+
+// a value below B p brought below 8p (B <= 16): one conditional subtraction
+// of 8p where B exceeds 8
+template <int B, typename W>
+__device__ __forceinline__ W red8(W s, const Bfly<W> &k)
+{
+    static_assert(B <= 16, "lazy bound");
+    if constexpr (B > 8)
+        return csub(s, k.p8, k.np8);
+    return s;
+}
 
 // x*w mod p in [0, 4p) for any 64-bit x: q' = xh*qh + hi(xl*qh) + hi(xh*ql)
 // is the Shoup quotient minus 0, 1 or 2, and x*w - q'*p = x*w + q'*(-p) mod 2^64.
@@ -309,105 +350,137 @@ __device__ __forceinline__ uint64_t shoup4(uint64_t x, uint64_t w, uint64_t wq, 
     return ((uint64_t)hi << 32) | (uint32_t)r;
 }
 
-// a + c reduced from [0, 8p) to [0, 4p): the candidate s - 4p is formed
-// first and selected, so no carry chain feeds the select
-__device__ __forceinline__ uint64_t add_4p(uint64_t a, uint64_t c, uint64_t p4, uint64_t np4)
+// Lazy forward rule.  Bound (in multiples of p) of register j of a forward
+// pass of R stages, before the stage on local bit lb (lb = -1: after the
+// pass), for inputs below b0 p.  A stage pairs j0 and j0 + 2^lb with the
+// bound b of the larger input: the sum is below 2b p (brought below 8p when
+// 2b > 8), the difference a - c + b p goes through the Shoup product (below
+// 4p), except for twiddle 1 (in the last pass, j0 = 0 mod 2^lb), where it is
+// kept like a sum.
+constexpr int fwd_bound(int j, int lb, int R, int b0, bool last)
 {
-    const uint64_t s = a + c;
-    const uint64_t t = s + np4;
-    return s >= p4 ? t : s;
+    if (lb == R - 1)
+        return b0;
+    const int pb = lb + 1;
+    const int j0 = j & ~(1 << pb);
+    const int a = fwd_bound(j0, pb, R, b0, last);
+    const int c = fwd_bound(j0 | (1 << pb), pb, R, b0, last);
+    const int b = a > c ? a : c;
+    const bool prod = ((j >> pb) & 1) && !(last && (j0 & ((1 << pb) - 1)) == 0);
+    return prod ? 4 : (2 * b > 8 ? 8 : 2 * b);
 }
 
-// a - c + 4p in (0, 8p) as a + ~c + (4p + 1): two 64-bit adds, no borrow chain
-__device__ __forceinline__ uint64_t sub_4p(uint64_t a, uint64_t c, uint64_t p4)
+constexpr int fwd_pass_out(int R, int b0, bool last)
 {
-    return a + ~c + (p4 + 1);
+    int m = 0;
+    for (int j = 0; j < (1 << R); ++j) {
+        const int b = fwd_bound(j, -1, R, b0, last);
+        m = b > m ? b : m;
+    }
+    return m;
 }
 
-template <typename W, bool LOOSE>
-__device__ __forceinline__ void bfly_dif(W &a, W &c, W w, W wq, const Bfly<W> &k)
+// Lazy inverse rule.  Bound of register j of an inverse pass before the
+// stage on local bit lb (stages run lb = 0 .. R-1; lb = R: after the pass).
+// A stage pairs j0 (upper, bound a) and j0 + 2^lb (bound c): v = c w is
+// below 4p, or for twiddle 1 (first inverse pass, j0 = 0 mod 2^lb) v = c
+// brought below 8p; with V the matching multiple (4 or 8), the outputs
+// a + v and a - v + V p are below (a' + V) p, a' = a unless a + V > 16, in
+// which case a is first brought below 8p.
+constexpr int inv_v(int c, bool triv)
 {
-    if constexpr (LOOSE) {
-        const W d = sub_4p(a, c, k.p4);
-        a = add_4p(a, c, k.p4, k.np4);
+    return triv ? (c <= 4 ? 4 : 8) : 4;
+}
+
+constexpr int inv_a(int a, int c, bool triv)
+{
+    return a + inv_v(c, triv) > 16 ? 8 : a;
+}
+
+constexpr int inv_bound(int j, int lb, int b0, bool first)
+{
+    if (lb == 0)
+        return b0;
+    const int pb = lb - 1;
+    const int j0 = j & ~(1 << pb);
+    const int a = inv_bound(j0, pb, b0, first);
+    const int c = inv_bound(j0 | (1 << pb), pb, b0, first);
+    const bool triv = first && (j0 & ((1 << pb) - 1)) == 0;
+    return inv_a(a, c, triv) + inv_v(c, triv);
+}
+
+constexpr int inv_pass_out(int R, int b0, bool first)
+{
+    int m = 0;
+    for (int j = 0; j < (1 << R); ++j) {
+        const int b = inv_bound(j, R, b0, first);
+        m = b > m ? b : m;
+    }
+    return m;
+}
+
+// Strict butterflies ([0, 2p) in and out).
+template <typename W>
+__device__ __forceinline__ void bfly_dif_strict(W &a, W &c, W w, W wq, const Bfly<W> &k)
+{
+    const W u = a, v = c;
+    a = add_2p<W>(u, v, k.p2);
+    c = shoup_lazy(sub_2p<W>(u, v, k.p2), w, wq, k.p);
+}
+
+template <typename W>
+__device__ __forceinline__ void bfly_dif1_strict(W &a, W &c, const Bfly<W> &k)
+{
+    const W u = a, v = c;
+    a = add_2p<W>(u, v, k.p2);
+    const W d = u + k.p2 - v;
+    c = d >= k.p2 ? d - k.p2 : d;
+}
+
+template <typename W>
+__device__ __forceinline__ void bfly_dit_strict(W &a, W &c, W w, W wq, const Bfly<W> &k)
+{
+    const W u = a;
+    const W v = shoup_lazy(c, w, wq, k.p);
+    a = add_2p<W>(u, v, k.p2);
+    const W d = u + k.p2 - v;
+    c = d >= k.p2 ? d - k.p2 : d;
+}
+
+// Lazy forward butterfly on inputs below B p (B <= 8); TRIV: twiddle 1.
+template <int B, bool TRIV>
+__device__ __forceinline__ void bfly_dif_lazy(uint64_t &a, uint64_t &c, uint64_t w, uint64_t wq,
+                                              const Bfly<uint64_t> &k)
+{
+    const uint64_t d = a - c + mult_p<B>(k);  // in (0, 2B p)
+    a = red8<2 * B>(a + c, k);
+    if constexpr (TRIV)
+        c = red8<2 * B>(d, k);
+    else
         c = shoup4(d, w, wq, k.np);
-    } else {
-        const W u = a, v = c;
-        a = add_2p<W>(u, v, k.p2);
-        c = shoup_lazy(sub_2p<W>(u, v, k.p2), w, wq, k.p);
-    }
 }
 
-// DIF butterfly whose twiddle is 1: (a + c, a - c), both kept in [0, 4p)
-// (loose) or [0, 2p) (strict)
-template <typename W, bool LOOSE>
-__device__ __forceinline__ void bfly_dif1(W &a, W &c, const Bfly<W> &k)
+// Lazy inverse butterfly on a below BA p and c below BC p.
+template <int BA, int BC, bool TRIV>
+__device__ __forceinline__ void bfly_dit_lazy(uint64_t &a, uint64_t &c, uint64_t w, uint64_t wq,
+                                              const Bfly<uint64_t> &k)
 {
-    if constexpr (LOOSE) {
-        W d = sub_4p(a, c, k.p4);
-        d = d >= k.p4 ? d + k.np4 : d;
-        a = add_4p(a, c, k.p4, k.np4);
-        c = d;
-    } else {
-        const W u = a, v = c;
-        a = add_2p<W>(u, v, k.p2);
-        const W d = u + k.p2 - v;
-        c = d >= k.p2 ? d - k.p2 : d;
-    }
+    constexpr int V = inv_v(BC, TRIV);
+    const uint64_t v = TRIV ? red8<BC>(c, k) : shoup4(c, w, wq, k.np);
+    const uint64_t ar = BA + V > 16 ? red8<BA>(a, k) : a;
+    a = ar + v;
+    c = ar - v + mult_p<V>(k);
 }
 
-// Inverse (DIT) butterflies.  Loose mode keeps the inverse's values in
-// [0, 8p) (8p < 2^64 for p < 2^61): the upper input a is brought to [0, 4p)
-// once, the product v is in [0, 4p) for any 64-bit c, so a + v and a - v + 4p
-// are back in [0, 8p) with one conditional step per butterfly.
-template <typename W, bool LOOSE>
-__device__ __forceinline__ void bfly_dit(W &a, W &c, W w, W wq, const Bfly<W> &k)
-{
-    if constexpr (LOOSE) {
-        const W ar = a >= k.p4 ? a + k.np4 : a;
-        const W v = shoup4(c, w, wq, k.np);
-        a = ar + v;
-        c = sub_4p(ar, v, k.p4);
-    } else {
-        const W u = a;
-        const W v = shoup_lazy(c, w, wq, k.p);
-        a = add_2p<W>(u, v, k.p2);
-        const W d = u + k.p2 - v;
-        c = d >= k.p2 ? d - k.p2 : d;
-    }
-}
-
-// inverse butterfly whose twiddle is 1: loose inputs in [0, 8p) are brought
-// to [0, 4p) first, so (a + c, a - c + 4p) stay in [0, 8p)
-template <typename W, bool LOOSE>
-__device__ __forceinline__ void bfly_dit1(W &a, W &c, const Bfly<W> &k)
-{
-    if constexpr (LOOSE) {
-        const W ar = a >= k.p4 ? a + k.np4 : a;
-        const W cr = c >= k.p4 ? c + k.np4 : c;
-        a = ar + cr;
-        c = sub_4p(ar, cr, k.p4);
-    } else {
-        bfly_dif1<W, false>(a, c, k);
-    }
-}
-
-// [0, 4p) (loose) or [0, 2p) (strict) -> [0, p)
-template <typename W, bool LOOSE>
+// [0, 2p) (strict) or below B p (lazy, B <= 8) -> [0, p)
+template <typename W, int B>
 __device__ __forceinline__ W bfly_final(W x, const Bfly<W> &k)
 {
-    if constexpr (LOOSE)
-        x = x >= k.p2 ? x - k.p2 : x;
-    return x >= k.p ? x - k.p : x;
-}
-
-// the inverse's [0, 8p) (loose) or [0, 2p) (strict) -> [0, p)
-template <typename W, bool LOOSE>
-__device__ __forceinline__ W bfly_final_inv(W x, const Bfly<W> &k)
-{
-    if constexpr (LOOSE)
-        x = x >= k.p4 ? x + k.np4 : x;
-    return bfly_final<W, LOOSE>(x, k);
+    if constexpr (B > 4)
+        x = csub(x, k.p4, k.np4);
+    if constexpr (B > 2)
+        x = csub(x, k.p2, k.np2);
+    return csub(x, k.p, k.np);
 }
 
 template <int L, int LOGE, int S0>
@@ -467,76 +540,145 @@ __device__ __forceinline__ void load_pass_tw(W (&tw)[2 * ((1 << LOGE) - 1)], int
     }
 }
 
-// forward (DIF) stages of the pass at S0, local bit LB down to 0
-template <typename W, bool LOOSE, int L, int LOGE, int S0, int LB>
+// forward (DIF) pair (J, J + 2^LB) of every group at the stage on local bit
+// LB of the pass at S0, whose registers were below B0 p at the pass start
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0, int LB, int J>
+__device__ __forceinline__ void fwd_pairs(W (&x)[1 << LOGE], const W (&tw)[2 * ((1 << LOGE) - 1)],
+                                          const Bfly<W> &k)
+{
+    using PS = PassShape<L, LOGE, S0>;
+    constexpr int R = PS::R;
+    if constexpr ((J & (1 << LB)) == 0) {
+        constexpr int jl = J & ((1 << LB) - 1);
+        // in the last pass (BLO = 0) the stage-table index is jl itself, and
+        // entry 0 of every stage table is w^0 = 1
+        constexpr bool TRIV = PS::BLO == 0 && jl == 0;
+        const W w = tw[2 * ((1 << LB) - 1 + jl)], wq = tw[2 * ((1 << LB) - 1 + jl) + 1];
+#pragma unroll
+        for (int g = 0; g < PS::G; ++g) {
+            W &a = x[g * (1 << R) + J];
+            W &c = x[g * (1 << R) + J + (1 << LB)];
+            if constexpr (LOOSE) {
+                constexpr int BA = fwd_bound(J, LB, R, B0, PS::BLO == 0);
+                constexpr int BC = fwd_bound(J + (1 << LB), LB, R, B0, PS::BLO == 0);
+                bfly_dif_lazy<(BA > BC ? BA : BC), TRIV>(a, c, w, wq, k);
+            } else if constexpr (TRIV) {
+                bfly_dif1_strict<W>(a, c, k);
+            } else {
+                bfly_dif_strict<W>(a, c, w, wq, k);
+            }
+        }
+    }
+    if constexpr (J + 1 < (1 << R))
+        fwd_pairs<W, LOOSE, L, LOGE, S0, B0, LB, J + 1>(x, tw, k);
+}
+
+// forward stages of the pass at S0, local bit LB down to 0
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0, int LB>
 __device__ __forceinline__ void fwd_stages(W (&x)[1 << LOGE], const W (&tw)[2 * ((1 << LOGE) - 1)],
                                            const Bfly<W> &k)
 {
-    using PS = PassShape<L, LOGE, S0>;
-    constexpr int R = PS::R;
+    fwd_pairs<W, LOOSE, L, LOGE, S0, B0, LB, 0>(x, tw, k);
+    if constexpr (LB > 0)
+        fwd_stages<W, LOOSE, L, LOGE, S0, B0, LB - 1>(x, tw, k);
+}
+
+// First stage of the first pass when the upper half of the input is zero:
+// the pair (a, 0) becomes (a, a w).  The outputs stay within the bounds of
+// the general stage (a below B0 p, a w below 4p, or 2p in strict mode).
+template <typename W, bool LOOSE, int L, int LOGE, int J>
+__device__ __forceinline__ void fwd_half_pairs(W (&x)[1 << LOGE],
+                                               const W (&tw)[2 * ((1 << LOGE) - 1)],
+                                               const Bfly<W> &k)
+{
+    using PS = PassShape<L, LOGE, 0>;
+    constexpr int LB = PS::R - 1;
+    constexpr bool TRIV = PS::BLO == 0 && J == 0;
+    const W w = tw[2 * ((1 << LB) - 1 + J)], wq = tw[2 * ((1 << LB) - 1 + J) + 1];
 #pragma unroll
     for (int g = 0; g < PS::G; ++g) {
+        const W a = x[g * (1 << PS::R) + J];
+        W &c = x[g * (1 << PS::R) + J + (1 << LB)];
+        if constexpr (TRIV)
+            c = a;
+        else if constexpr (LOOSE)
+            c = shoup4(a, w, wq, k.np);
+        else
+            c = shoup_lazy(a, w, wq, k.p);
+    }
+    if constexpr (J + 1 < (1 << LB))
+        fwd_half_pairs<W, LOOSE, L, LOGE, J + 1>(x, tw, k);
+}
+
+// inverse (DIT) pair (J, J + 2^LB) of every group, stage LB of the pass at
+// S0, whose registers were below B0 p at the pass start
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0, int LB, int J>
+__device__ __forceinline__ void inv_pairs(W (&x)[1 << LOGE], const W (&tw)[2 * ((1 << LOGE) - 1)],
+                                          const Bfly<W> &k)
+{
+    using PS = PassShape<L, LOGE, S0>;
+    constexpr int R = PS::R;
+    if constexpr ((J & (1 << LB)) == 0) {
+        constexpr int jl = J & ((1 << LB) - 1);
+        constexpr bool TRIV = PS::BLO == 0 && jl == 0;  // twiddle w^0 = 1 (see fwd_pairs)
+        const W w = tw[2 * ((1 << LB) - 1 + jl)], wq = tw[2 * ((1 << LB) - 1 + jl) + 1];
 #pragma unroll
-        for (int jl = 0; jl < (1 << LB); ++jl) {
-            const W w = tw[2 * ((1 << LB) - 1 + jl)], wq = tw[2 * ((1 << LB) - 1 + jl) + 1];
-#pragma unroll
-            for (int jh = 0; jh < (1 << (R - 1 - LB)); ++jh) {
-                const int j = (jh << (LB + 1)) | jl;
-                // in the last pass (BLO = 0) the stage-table index is jl itself,
-                // and entry 0 of every stage table is w^0 = 1
-                if (PS::BLO == 0 && jl == 0)
-                    bfly_dif1<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], k);
-                else
-                    bfly_dif<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
+        for (int g = 0; g < PS::G; ++g) {
+            W &a = x[g * (1 << R) + J];
+            W &c = x[g * (1 << R) + J + (1 << LB)];
+            if constexpr (LOOSE) {
+                constexpr int BA = inv_bound(J, LB, B0, PS::BLO == 0);
+                constexpr int BC = inv_bound(J + (1 << LB), LB, B0, PS::BLO == 0);
+                bfly_dit_lazy<BA, BC, TRIV>(a, c, w, wq, k);
+            } else if constexpr (TRIV) {
+                bfly_dif1_strict<W>(a, c, k);  // (a + c, a - c) either way
+            } else {
+                bfly_dit_strict<W>(a, c, w, wq, k);
             }
         }
     }
-    if constexpr (LB > 0)
-        fwd_stages<W, LOOSE, L, LOGE, S0, LB - 1>(x, tw, k);
+    if constexpr (J + 1 < (1 << R))
+        inv_pairs<W, LOOSE, L, LOGE, S0, B0, LB, J + 1>(x, tw, k);
 }
 
-// inverse (DIT) stages of the pass at S0, local bit LB up to R-1
-template <typename W, bool LOOSE, int L, int LOGE, int S0, int LB>
+// inverse stages of the pass at S0, local bit LB up to R-1
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0, int LB>
 __device__ __forceinline__ void inv_stages(W (&x)[1 << LOGE], const W (&tw)[2 * ((1 << LOGE) - 1)],
                                            const Bfly<W> &k)
 {
-    using PS = PassShape<L, LOGE, S0>;
-    constexpr int R = PS::R;
-#pragma unroll
-    for (int g = 0; g < PS::G; ++g) {
-#pragma unroll
-        for (int jl = 0; jl < (1 << LB); ++jl) {
-            const W w = tw[2 * ((1 << LB) - 1 + jl)], wq = tw[2 * ((1 << LB) - 1 + jl) + 1];
-#pragma unroll
-            for (int jh = 0; jh < (1 << (R - 1 - LB)); ++jh) {
-                const int j = (jh << (LB + 1)) | jl;
-                if (PS::BLO == 0 && jl == 0)  // twiddle w^0 = 1 (see fwd_stages)
-                    bfly_dit1<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], k);
-                else
-                    bfly_dit<W, LOOSE>(x[g * (1 << R) + j], x[g * (1 << R) + j + (1 << LB)], w, wq, k);
-            }
-        }
-    }
-    if constexpr (LB + 1 < R)
-        inv_stages<W, LOOSE, L, LOGE, S0, LB + 1>(x, tw, k);
+    inv_pairs<W, LOOSE, L, LOGE, S0, B0, LB, 0>(x, tw, k);
+    if constexpr (LB + 1 < PassShape<L, LOGE, S0>::R)
+        inv_stages<W, LOOSE, L, LOGE, S0, B0, LB + 1>(x, tw, k);
 }
 
 // forward passes from S0 to the end (x holds the pass-S0 arrangement and tw
-// its twiddles); on return the data sits in LDS in natural (DIF output) order
-template <typename W, bool LOOSE, int L, int LOGE, int S0>
+// its twiddles, registers below B0 p); `half`: the upper half of the input
+// is zero (first pass only).  On return the last pass's registers hold runs
+// of consecutive positions, below fwd_bound(j, -1, R, fwd_last_b0) p.
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0>
 __device__ __forceinline__ void fwd_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 << LOGE) - 1)],
                                            W *lds, int t, const W *__restrict__ stw,
-                                           const Bfly<W> &k)
+                                           const Bfly<W> &k, bool half)
 {
     using PS = PassShape<L, LOGE, S0>;
-    fwd_stages<W, LOOSE, L, LOGE, S0, PS::R - 1>(x, tw, k);
+    if constexpr (S0 == 0) {
+        if (half)
+            fwd_half_pairs<W, LOOSE, L, LOGE, 0>(x, tw, k);
+        else
+            fwd_pairs<W, LOOSE, L, LOGE, 0, B0, PS::R - 1, 0>(x, tw, k);
+        if constexpr (PS::R > 1)
+            fwd_stages<W, LOOSE, L, LOGE, 0, B0, PS::R - 2>(x, tw, k);
+    } else {
+        fwd_stages<W, LOOSE, L, LOGE, S0, B0, PS::R - 1>(x, tw, k);
+    }
     if constexpr (S0 + PS::R < L) {
         load_pass_tw<W, L, LOGE, S0 + PS::R>(tw, t, stw);  // in flight during the exchange
         lds_store<W, L, LOGE, S0>(x, lds, t);
         __syncthreads();
         lds_load<W, L, LOGE, S0 + PS::R>(x, lds, t);
         __syncthreads();
-        fwd_passes<W, LOOSE, L, LOGE, S0 + PS::R>(x, tw, lds, t, stw, k);
+        constexpr int B1 = LOOSE ? fwd_pass_out(PS::R, B0, false) : 2;
+        fwd_passes<W, LOOSE, L, LOGE, S0 + PS::R, B1>(x, tw, lds, t, stw, k, false);
     }
 }
 
@@ -545,6 +687,26 @@ template <int L, int LOGE>
 constexpr int last_pass_start()
 {
     return ((L - 1) / LOGE) * LOGE;
+}
+
+// register bound at the start of the last forward pass, for inputs below b0 p
+template <int L, int LOGE>
+constexpr int fwd_last_b0(int b0)
+{
+    int b = b0;
+    for (int s0 = 0; s0 < last_pass_start<L, LOGE>(); s0 += LOGE)
+        b = fwd_pass_out(LOGE, b, false);
+    return b;
+}
+
+// lazy registers of the last pass (run length 2^R) brought to [0, p)
+template <typename W, bool LOOSE, int LOGE, int R, int B0, int J>
+__device__ __forceinline__ void fwd_final(const W (&x)[1 << LOGE], int base, W *y, const Bfly<W> &k)
+{
+    constexpr int B = LOOSE ? fwd_bound(J, -1, R, B0, true) : 2;
+    y[J] = bfly_final<W, B>(x[base + J], k);
+    if constexpr (J + 1 < (1 << R))
+        fwd_final<W, LOOSE, LOGE, R, B0, J + 1>(x, base, y, k);
 }
 
 // CNT consecutive words (CNT a power of two, 16-byte aligned runs when
@@ -602,20 +764,25 @@ __device__ __forceinline__ void load_words(const W *src, W *v)
     }
 }
 
-// inverse passes from S0 down to 0; on return x holds the pass-0 arrangement
-template <typename W, bool LOOSE, int L, int LOGE, int S0>
+// inverse passes from S0 down to 0 (registers below B0 p at the start of
+// the pass at S0); on return x holds the pass-0 arrangement, below
+// inv_pass_out(...) p in lazy mode (any 64-bit value is a valid input of the
+// final scaling or of the merge pass's twist)
+template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0>
 __device__ __forceinline__ void inv_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 << LOGE) - 1)],
                                            W *lds, int t, const W *__restrict__ sitw,
                                            const Bfly<W> &k)
 {
-    inv_stages<W, LOOSE, L, LOGE, S0, 0>(x, tw, k);
+    using PS = PassShape<L, LOGE, S0>;
+    inv_stages<W, LOOSE, L, LOGE, S0, B0, 0>(x, tw, k);
     if constexpr (S0 > 0) {
         load_pass_tw<W, L, LOGE, S0 - LOGE>(tw, t, sitw);
         __syncthreads();  // every thread has finished reading the previous arrangement
         lds_store<W, L, LOGE, S0>(x, lds, t);
         __syncthreads();
         lds_load<W, L, LOGE, S0 - LOGE>(x, lds, t);
-        inv_passes<W, LOOSE, L, LOGE, S0 - LOGE>(x, tw, lds, t, sitw, k);
+        constexpr int B1 = LOOSE ? inv_pass_out(PS::R, B0, PS::BLO == 0) : 2;
+        inv_passes<W, LOOSE, L, LOGE, S0 - LOGE, B1>(x, tw, lds, t, sitw, k);
     }
 }
 
@@ -667,7 +834,11 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
         x[j] = v;
     }
     const Bfly<W> k(p);
-    fwd_passes<W, LOOSE, L, LOGE, 0>(x, tw, lds, t, tab.stw, k);
+    // inputs below 2p (coefficients, the column-in-load sums, or the column
+    // pass's reduced rows); a direct load of at most M/2 coefficients leaves
+    // the upper half zero
+    fwd_passes<W, LOOSE, L, LOGE, 0, 2>(x, tw, lds, t, tab.stw, k,
+                                        !pre && R == 1 && in_len <= (M >> 1));
     // the last pass leaves each thread runs of 2^R consecutive positions:
     // store them from registers (no LDS round trip)
     const int64_t count = units >> logR;
@@ -679,9 +850,7 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
 #pragma unroll
     for (int g = 0; g < PSL::G; ++g) {
         W y[RUN];
-#pragma unroll
-        for (int j = 0; j < RUN; ++j)
-            y[j] = bfly_final<W, LOOSE>(x[g * RUN + j], k);
+        fwd_final<W, LOOSE, LOGE, PSL::R, fwd_last_b0<L, LOGE>(2), 0>(x, g * RUN, y, k);
         if (logP >= 2) {
 #pragma unroll
             for (int j0 = 0; j0 < RUN; j0 += VEC)
@@ -744,7 +913,8 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
         }
     }
     const Bfly<W> k(p);
-    inv_passes<W, LOOSE, L, LOGE, SLAST>(x, tw, lds, t, tab.sitw, k);
+    // inputs below 8p (the pointwise products' lazy outputs)
+    inv_passes<W, LOOSE, L, LOGE, SLAST, 8>(x, tw, lds, t, tab.sitw, k);
     // pass-0 arrangement: element j is index (j << (L-LOGE)) | t
     if (R == 1) {
         Wout *o = out + e * out_stride;
@@ -755,10 +925,12 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
                 o[i1 - lo] = (Wout)shoup<W>(x[j], tab.ninv, tab.ninvq, p);
         }
     } else {
+        // lazy values: the merge pass starts with a Shoup product (twist),
+        // whose output is in [0, 2p) for any word
         W *o = tmp + e * N + (int64_t)gg * M;
 #pragma unroll
         for (int j = 0; j < E; ++j)
-            o[(j << (L - LOGE)) | t] = bfly_final_inv<W, LOOSE>(x[j], k);
+            o[(j << (L - LOGE)) | t] = x[j];
     }
 }
 
@@ -848,7 +1020,7 @@ hipError_t launch_fwd_reg(int logE, const Win *in, int64_t in_stride, int64_t in
                           hipStream_t s, int pre)
 {
     if constexpr (sizeof(W) == 8) {
-        if ((uint64_t)p < ((uint64_t)1 << 61))
+        if ((uint64_t)p < ((uint64_t)1 << 60))
             return launch_fwd_reg2<Win, W, true, L>(logE, in, in_stride, in_len, out, units, logR,
                                                     logP, tab, p, s, pre);
     }
@@ -862,7 +1034,7 @@ hipError_t launch_inv_reg(int logE, const W *in, Wout *out, int64_t out_stride, 
                           const NttTab<W> &tab, W p, hipStream_t s)
 {
     if constexpr (sizeof(W) == 8) {
-        if ((uint64_t)p < ((uint64_t)1 << 61))
+        if ((uint64_t)p < ((uint64_t)1 << 60))
             return launch_inv_reg2<W, Wout, true, L>(logE, in, out, out_stride, lo, hi, tmp, units,
                                                      logR, logP, tab, p, s);
     }
@@ -1315,7 +1487,7 @@ hipError_t fwd_segs_reg(int logE, const FwdSeg<Win, W> *segs, int nseg, int64_t 
                         int logP, const NttTab<W> &tab, W p, hipStream_t s)
 {
     if constexpr (sizeof(W) == 8) {
-        if ((uint64_t)p < ((uint64_t)1 << 61))
+        if ((uint64_t)p < ((uint64_t)1 << 60))
             return fwd_segs_reg2<Win, W, true, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
     }
     return fwd_segs_reg2<Win, W, false, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
@@ -1326,7 +1498,7 @@ hipError_t inv_segs_reg(int logE, const InvSeg<W, Wout> *segs, int nseg, int64_t
                         int logP, const NttTab<W> &tab, W p, hipStream_t s)
 {
     if constexpr (sizeof(W) == 8) {
-        if ((uint64_t)p < ((uint64_t)1 << 61))
+        if ((uint64_t)p < ((uint64_t)1 << 60))
             return inv_segs_reg2<W, Wout, true, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
     }
     return inv_segs_reg2<W, Wout, false, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
