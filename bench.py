@@ -45,11 +45,14 @@ GOLDEN_FILE = os.path.join(ROOT, "tests", "golden", "golden.json")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=16, help="products per step")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="time budget of the CPU baseline sample")
+    ap.add_argument("--config", type=int, default=None,
+                    help="run the batch in this product configuration (0..7) instead of the "
+                         "learned dispatch's pick, e.g. so that profiling runs see one set of kernels")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the comparison forms")
     ap.add_argument("--no-extra", action="store_true",
@@ -255,9 +258,14 @@ def main():
     pml_amd.set_refine(False)
     model_choice = pml_amd.choose_config(ctx, M, K, N_, LEN, LEN, "auto")
     pml_amd.set_refine(True)
-    chosen = pml_amd.choose_config(ctx, M, K, N_, LEN, LEN, "auto")
-
-    plan = pml_amd.BatchPlan([(ctx, As[i], Bs[i]) for i in range(Bn)], method="auto", graph=True)
+    if args.config is None:
+        chosen = pml_amd.choose_config(ctx, M, K, N_, LEN, LEN, "auto")
+        plan = pml_amd.BatchPlan([(ctx, As[i], Bs[i]) for i in range(Bn)], method="auto",
+                                 graph=True)
+    else:
+        chosen = args.config
+        plan = pml_amd.BatchPlan([(ctx, As[i], Bs[i]) for i in range(Bn)], config=chosen,
+                                 graph=True)
     # graph instances with events between the stages, one per timed step: the
     # stage times below come from the timed replays themselves
     plan.time_stages(min(args.steps, 256))
@@ -370,7 +378,9 @@ def main():
         "data": "synthetic (seeded uniform residues mod p)",
         "config": {"workload": "polmat_mul 32x32*32x32, length 4096 each, p=2^60-3*2^24+1, "
                                "NTT length 8192, 8192 pointwise 32x32x32 products per product",
-                   "batch": Bn, "dispatch": "learned+refined", "config_chosen": chosen,
+                   "batch": Bn,
+                   "dispatch": "learned+refined" if args.config is None else "forced",
+                   "config_chosen": chosen,
                    "config_model": model_choice,
                    "parity_entry00": ok00, "parity_eval_identity_all_entries": okr},
         "forms_products_per_s": {k: round(v, 3) for k, v in forms.items()},

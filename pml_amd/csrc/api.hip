@@ -289,8 +289,7 @@ pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, v
     uint64_t *tmpf = static_cast<uint64_t *>(tmpv);    // forward column pass
     const int64_t N = (int64_t)1 << logN;
     const W p = (W)ctx->p;
-    // point-blocked evaluation layout: 32-byte runs per entry
-    const int logP = std::min(sizeof(W) == 8 ? 2 : 3, logN);
+    const int logP = eval_log_run_in<W>(logN), logPc = eval_log_run_out<W>(logN);
     const NttTune &tune = ctx->tune;
     Prof pr;
     pr.stage = -1;
@@ -320,13 +319,13 @@ pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, v
     }
     if (prof)
         prof_begin(ctx, 1, s, &pr);
-    HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, N, logP, mc, s),
+    HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, N, logP, logPc, mc, s),
             "pointwise");
     if (prof) {
         prof_end(ctx, s, &pr);
         prof_begin(ctx, 2, s, &pr);
     }
-    HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, logP, tune,
+    HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, logPc, tune,
                                          itab, p, tmp, s, ct)),
             "ntt_inv");
     if (prof)

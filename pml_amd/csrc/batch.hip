@@ -498,7 +498,7 @@ void add_segments(const pmlg_mul_desc &d, const ProdInfo &pi, unsigned char *are
     const ProdConfig &cf = kConfigs[pi.config];
     const int logN = pi.logN;
     const int64_t N = (int64_t)1 << logN;
-    const int logP = std::min(sizeof(W) == 8 ? 2 : 3, logN);
+    const int logP = eval_log_run_in<W>(logN);
     W *Ah = reinterpret_cast<W *>(arena) + pi.ah;
     W *Bh = reinterpret_cast<W *>(arena) + pi.bh;
     W *Ch = reinterpret_cast<W *>(arena) + pi.ch;
@@ -523,6 +523,7 @@ void add_segments(const pmlg_mul_desc &d, const ProdInfo &pi, unsigned char *are
     ps.k = (int32_t)d.k;
     ps.n = (int32_t)d.n;
     ps.logP = logP;
+    ps.logPc = eval_log_run_out<W>(logN);
     ps.N = N;
     ps.redc_small = 0;
     if (redc) {
@@ -551,7 +552,7 @@ void finish_groups(pmlg_batch_plan plan, Blob &blob,
             sg.block0 = b;
             b += ntt_seg_blocks<W>(sg.count, logN);
         }
-        FwdGroup g{std::get<0>(kv.first), logN, std::min(sizeof(W) == 8 ? 2 : 3, logN),
+        FwdGroup g{std::get<0>(kv.first), logN, eval_log_run_in<W>(logN),
                    std::get<2>(kv.first), blob.add(kv.second), (int)kv.second.size(), b};
         plan->fwd.push_back(g);
     }
@@ -564,7 +565,7 @@ void finish_groups(pmlg_batch_plan plan, Blob &blob,
             b += ntt_seg_blocks<W>(sg.count, logN);
             mb += merge_seg_blocks<W>(sg.count, sg.hi - sg.lo, logN);
         }
-        InvGroup g{std::get<0>(kv.first), logN, std::min(sizeof(W) == 8 ? 2 : 3, logN),
+        InvGroup g{std::get<0>(kv.first), logN, eval_log_run_out<W>(logN),
                    std::get<2>(kv.first), std::get<3>(kv.first) != 0, blob.add(kv.second),
                    (int)kv.second.size(), b, mb};
         plan->inv.push_back(g);

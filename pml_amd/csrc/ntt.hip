@@ -24,16 +24,16 @@ constexpr int kInvWaves = PMLG_INV_WAVES;  // minimum waves per SIMD of the inve
 
 // Point-blocked evaluation layout: position `pos` of entry `e` (of `count`
 // entries) lives at ((pos >> logP) * count + e) * P + (pos mod P), P = 2^logP,
-// so that the P values of every entry at P consecutive points form one 32-byte
-// run and all entries of one point block are contiguous.
+// so that the P values of every entry at P consecutive points form one run
+// and all entries of one point block are contiguous.
 __device__ __forceinline__ int64_t blk_addr(int64_t e, int64_t pos, int64_t count, int logP)
 {
     return (((pos >> logP) * count + e) << logP) | (pos & ((1 << logP) - 1));
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give the workgroups
-// of one XCD consecutive units so that neighbouring entries (which share
-// 128-byte lines in the blocked layout) meet in the same L2.
+// of one XCD consecutive units, so that neighbouring entries (which share
+// lines when the runs are shorter than a line) meet in the same L2.
 __device__ __forceinline__ int64_t xcd_unit(int64_t b, int64_t units)
 {
     return (units & 7) ? b : (b & 7) * (units >> 3) + (b >> 3);
@@ -845,7 +845,7 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
     using PSL = PassShape<L, LOGE, last_pass_start<L, LOGE>()>;
     constexpr int RUN = 1 << PSL::R;
     // words per vector store: within one layout block of 2^logP positions
-    // (logP >= 2, and for 32-bit words logP = 3 unless N < 8)
+    // (logP >= 2 unless N < 4)
     constexpr int VEC = RUN < 4 ? RUN : 4;
 #pragma unroll
     for (int g = 0; g < PSL::G; ++g) {

@@ -48,8 +48,8 @@ __device__ __forceinline__ void plain_fold(uint64_t &hi, uint64_t &lo, const Mod
 __device__ __forceinline__ void plain_u64_body(const uint64_t *__restrict__ Ah,
                                                const uint64_t *__restrict__ Bh,
                                                uint64_t *__restrict__ Ch, int m, int k, int n,
-                                               int64_t N, int logP, const ModConst &mc, int64_t q,
-                                               int tile)
+                                               int64_t N, int logP, int logPc, const ModConst &mc,
+                                               int64_t q, int tile)
 {
     const int tiles_n = (n + TN - 1) / TN;
     const int i0 = (tile / tiles_n) * TM;
@@ -90,15 +90,15 @@ __device__ __forceinline__ void plain_u64_body(const uint64_t *__restrict__ Ah,
 #pragma unroll
         for (int c = 0; c < TN; ++c)
             if (i0 + r < m && j0 + c < n)
-                Ch[blk_addr((int64_t)(i0 + r) * n + j0 + c, q, (int64_t)m * n, logP)] =
+                Ch[blk_addr((int64_t)(i0 + r) * n + j0 + c, q, (int64_t)m * n, logPc)] =
                     reduce128(hi[r][c], lo[r][c], mc);
 }
 
 __device__ __forceinline__ void plain_u32_body(const uint32_t *__restrict__ Ah,
                                                const uint32_t *__restrict__ Bh,
                                                uint32_t *__restrict__ Ch, int m, int k, int n,
-                                               int64_t N, int logP, const ModConst &mc, int64_t q,
-                                               int tile)
+                                               int64_t N, int logP, int logPc, const ModConst &mc,
+                                               int64_t q, int tile)
 {
     const int tiles_n = (n + TN - 1) / TN;
     const int i0 = (tile / tiles_n) * TM;
@@ -139,7 +139,7 @@ __device__ __forceinline__ void plain_u32_body(const uint32_t *__restrict__ Ah,
 #pragma unroll
         for (int c = 0; c < TN; ++c)
             if (i0 + r < m && j0 + c < n)
-                Ch[blk_addr((int64_t)(i0 + r) * n + j0 + c, q, (int64_t)m * n, logP)] =
+                Ch[blk_addr((int64_t)(i0 + r) * n + j0 + c, q, (int64_t)m * n, logPc)] =
                     (uint32_t)barrett(acc[r][c], mc.bm, mc.p);
 }
 
@@ -147,14 +147,14 @@ __device__ __forceinline__ void plain_u32_body(const uint32_t *__restrict__ Ah,
 template <typename W>
 __global__ void __launch_bounds__(kThreads)
 pw_plain(const W *__restrict__ Ah, const W *__restrict__ Bh, W *__restrict__ Ch, int m, int k,
-         int n, int64_t N, int logP, ModConst mc)
+         int n, int64_t N, int logP, int logPc, ModConst mc)
 {
     const int64_t q = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     const int tile = blockIdx.y * 4 + (threadIdx.x >> 6);
     if constexpr (sizeof(W) == 8)
-        plain_u64_body(Ah, Bh, Ch, m, k, n, N, logP, mc, q, tile);
+        plain_u64_body(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, q, tile);
     else
-        plain_u32_body(Ah, Bh, Ch, m, k, n, N, logP, mc, q, tile);
+        plain_u32_body(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, q, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -277,7 +277,7 @@ __device__ __forceinline__ int mfma_row(int r, int lane)
 template <typename W, int L, int PB>
 __device__ __forceinline__ void mfma_body(const W *__restrict__ Ah, const W *__restrict__ Bh,
                                           W *__restrict__ Ch, int m, int k, int n, int logP,
-                                          const ModConst &mc, int64_t blk)
+                                          int logPc, const ModConst &mc, int64_t blk)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     const int Mp = (m + 31) & ~31, Kp = (k + 31) & ~31, Np = (n + 31) & ~31;
@@ -325,8 +325,9 @@ __device__ __forceinline__ void mfma_body(const W *__restrict__ Ah, const W *__r
     constexpr int GW = (L == 8) ? 5 : 4;   // weights per partial sum (last one may be shorter)
 
     // output addressing: entry (row, col) of point q0+pt is at
-    // cbase + (row*n + col) * 2^logP with cbase fixed by the point block
-    const int64_t cbase = (((q0 >> logP) * ((int64_t)m * n)) << logP) | (q0 & ((1 << logP) - 1));
+    // cbase + (row*n + col) * 2^logPc with cbase fixed by the point block
+    // (PB <= 2^logP <= 2^logPc: the block lies within one run of C)
+    const int64_t cbase = (((q0 >> logPc) * ((int64_t)m * n)) << logPc) | (q0 & ((1 << logPc) - 1));
 
     for (int task = wave; task < ntask; task += kThreads / 64) {
         const int pt = task / (tilesM * tilesN);
@@ -389,8 +390,8 @@ __device__ __forceinline__ void mfma_body(const W *__restrict__ Ah, const W *__r
 
         const int col = nt * 32 + (lane & 31);
         const int row0 = mt * 32 + 4 * (lane >> 5);
-        const int nP = n << logP;  // row stride of C in the blocked layout (uniform, 32-bit)
-        W *cp = Ch + cbase + pt + (((int64_t)row0 * n + col) << logP);
+        const int nP = n << logPc;  // row stride of C in the blocked layout (uniform, 32-bit)
+        W *cp = Ch + cbase + pt + (((int64_t)row0 * n + col) << logPc);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int dr = 8 * (r >> 2) + (r & 3);  // mfma_row(r, lane) - 4 (lane / 32)
@@ -427,7 +428,7 @@ __device__ __forceinline__ void mfma_body(const W *__restrict__ Ah, const W *__r
 template <typename W, int L, int PB>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 pw_mfma_kernel(const W *__restrict__ Ah, const W *__restrict__ Bh, W *__restrict__ Ch, int m,
-               int k, int n, int64_t N, int logP, ModConst mc)
+               int k, int n, int64_t N, int logP, int logPc, ModConst mc)
 {
     // XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (round-robin
     // dispatch), so give them consecutive point blocks to share its L2
@@ -435,7 +436,7 @@ pw_mfma_kernel(const W *__restrict__ Ah, const W *__restrict__ Bh, W *__restrict
     int64_t blk = blockIdx.x;
     if ((nblk & 7) == 0)
         blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
-    mfma_body<W, L, PB>(Ah, Bh, Ch, m, k, n, logP, mc, blk);
+    mfma_body<W, L, PB>(Ah, Bh, Ch, m, k, n, logP, logPc, mc, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,7 +466,8 @@ pw_mfma_segs(const PwSeg<W> *__restrict__ segs, int nseg, int64_t blocks, ModCon
         const PwSeg<W> &sg = pw_seg_of(segs, nseg, b);
         ModConst mcs = mc;
         mcs.redc_small = sg.redc_small;
-        mfma_body<W, L, PB>(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.logP, mcs, b - sg.block0);
+        mfma_body<W, L, PB>(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.logP, sg.logPc, mcs,
+                            b - sg.block0);
         __syncthreads();  // the next task rewrites the limb planes
     }
 }
@@ -483,9 +485,9 @@ pw_plain_segs(const PwSeg<W> *__restrict__ segs, int nseg, int64_t blocks, ModCo
         const int64_t q = (lb - by * gx) * 64 + (threadIdx.x & 63);
         const int tile = (int)by * 4 + (threadIdx.x >> 6);
         if constexpr (sizeof(W) == 8)
-            plain_u64_body(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.N, sg.logP, mc, q, tile);
+            plain_u64_body(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.N, sg.logP, sg.logPc, mc, q, tile);
         else
-            plain_u32_body(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.N, sg.logP, mc, q, tile);
+            plain_u32_body(sg.Ah, sg.Bh, sg.Ch, sg.m, sg.k, sg.n, sg.N, sg.logP, sg.logPc, mc, q, tile);
     }
 }
 
@@ -497,7 +499,7 @@ inline int64_t mfma_point_bytes(int m, int k, int n, int limbs)
 
 template <typename W, int L, int PB>
 hipError_t launch_mfma_pb(const W *Ah, const W *Bh, W *Ch, int m, int k, int n, int64_t N,
-                          int logP, const ModConst &mc, hipStream_t s)
+                          int logP, int logPc, const ModConst &mc, hipStream_t s)
 {
     const size_t lds = (size_t)PB * mfma_point_bytes(m, k, n, L);
     static bool attr_set = false;
@@ -510,13 +512,13 @@ hipError_t launch_mfma_pb(const W *Ah, const W *Bh, W *Ch, int m, int k, int n, 
     }
     const int64_t blocks = N / PB;
     hipLaunchKernelGGL((pw_mfma_kernel<W, L, PB>), dim3((unsigned)blocks), dim3(kThreads), lds, s,
-                       Ah, Bh, Ch, m, k, n, N, logP, mc);
+                       Ah, Bh, Ch, m, k, n, N, logP, logPc, mc);
     return hipGetLastError();
 }
 
 template <typename W, int L>
 hipError_t launch_mfma(const W *Ah, const W *Bh, W *Ch, int m, int k, int n, int64_t N, int logP,
-                       const ModConst &mc, hipStream_t s)
+                       int logPc, const ModConst &mc, hipStream_t s)
 {
     // Points per workgroup: within one layout block, and at most 64 KiB of
     // limb planes so that two workgroups share a CU.
@@ -524,12 +526,12 @@ hipError_t launch_mfma(const W *Ah, const W *Bh, W *Ch, int m, int k, int n, int
     constexpr int64_t pref = 64 * 1024;
     const int64_t P = (int64_t)1 << logP;
     if (P >= 8 && 8 * per <= pref)
-        return launch_mfma_pb<W, L, 8>(Ah, Bh, Ch, m, k, n, N, logP, mc, s);
+        return launch_mfma_pb<W, L, 8>(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, s);
     if (P >= 4 && 4 * per <= pref)
-        return launch_mfma_pb<W, L, 4>(Ah, Bh, Ch, m, k, n, N, logP, mc, s);
+        return launch_mfma_pb<W, L, 4>(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, s);
     if (P >= 2 && 2 * per <= pref)
-        return launch_mfma_pb<W, L, 2>(Ah, Bh, Ch, m, k, n, N, logP, mc, s);
-    return launch_mfma_pb<W, L, 1>(Ah, Bh, Ch, m, k, n, N, logP, mc, s);
+        return launch_mfma_pb<W, L, 2>(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, s);
+    return launch_mfma_pb<W, L, 1>(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, s);
 }
 
 }  // namespace
@@ -552,34 +554,34 @@ bool mfma_supported(int m, int k, int n, int limbs, uint64_t p)
 template <>
 hipError_t launch_pointwise<uint64_t>(PointwiseKind kind, const uint64_t *Ah, const uint64_t *Bh,
                                       uint64_t *Ch, int m, int k, int n, int64_t N, int logP,
-                                      const ModConst &mc, hipStream_t s)
+                                      int logPc, const ModConst &mc, hipStream_t s)
 {
     if (kind == PW_MFMA) {
         if (mc.limbs != 8 || !mfma_supported(m, k, n, 8, mc.p))
             return hipErrorInvalidValue;
-        return launch_mfma<uint64_t, 8>(Ah, Bh, Ch, m, k, n, N, logP, mc, s);
+        return launch_mfma<uint64_t, 8>(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, s);
     }
     const int tiles = ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
     dim3 grid((unsigned)((N + 63) / 64), (unsigned)((tiles + 3) / 4));
     hipLaunchKernelGGL(pw_plain<uint64_t>, grid, dim3(kThreads), 0, s, Ah, Bh, Ch, m, k, n, N,
-                       logP, mc);
+                       logP, logPc, mc);
     return hipGetLastError();
 }
 
 template <>
 hipError_t launch_pointwise<uint32_t>(PointwiseKind kind, const uint32_t *Ah, const uint32_t *Bh,
                                       uint32_t *Ch, int m, int k, int n, int64_t N, int logP,
-                                      const ModConst &mc, hipStream_t s)
+                                      int logPc, const ModConst &mc, hipStream_t s)
 {
     if (kind == PW_MFMA) {
         if (mc.limbs != 4 || !mfma_supported(m, k, n, 4, mc.p))
             return hipErrorInvalidValue;
-        return launch_mfma<uint32_t, 4>(Ah, Bh, Ch, m, k, n, N, logP, mc, s);
+        return launch_mfma<uint32_t, 4>(Ah, Bh, Ch, m, k, n, N, logP, logPc, mc, s);
     }
     const int tiles = ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
     dim3 grid((unsigned)((N + 63) / 64), (unsigned)((tiles + 3) / 4));
     hipLaunchKernelGGL(pw_plain<uint32_t>, grid, dim3(kThreads), 0, s, Ah, Bh, Ch, m, k, n, N,
-                       logP, mc);
+                       logP, logPc, mc);
     return hipGetLastError();
 }
 
