@@ -31,7 +31,8 @@
  * for p >= 2^30, 2^26 for p < 2^30), the product is evaluated modulo p
  * itself; otherwise (non-FFT primes, composite moduli, p >= 2^62) it is
  * computed modulo three 60-bit FFT primes and recombined by CRT (N <= 2^24,
- * k min(lenA, lenB) < 2^51).  Approximant and kernel bases need an odd prime.
+ * k min(lenA, lenB) < 2^51).  Approximant and kernel bases need a prime modulus
+ * (p = 2 included).
  * Longer products return PMLG_ERR_UNSUPPORTED.
  */
 #ifndef PMLGPU_H
@@ -193,7 +194,7 @@ pmlg_status pmlg_pmbasis_leaf_order(pmlg_ctx ctx, int64_t m, int64_t n, int meth
  * basis, product and middle product runs on the device.  When capK < *lenK,
  * K is not written and PMLG_ERR_ARG is returned with *lenK and *nullity set:
  * call again with capK >= *lenK.  1 <= m <= 256; n <= 128 for
- * PMLG_KERNEL_VIA_APPROX; the modulus must be an odd prime. */
+ * PMLG_KERNEL_VIA_APPROX; the modulus must be prime (p = 2 included). */
 #define PMLG_KERNEL_ZLS 0
 #define PMLG_KERNEL_VIA_APPROX 1
 pmlg_status pmlg_kernel_basis(pmlg_ctx ctx, uint64_t *K, int64_t capK, int64_t *lenK,

@@ -1008,9 +1008,130 @@ static int has_form(const nmod_poly_mat_struct *M, const slong *shift, poly_mat_
     return ok;
 }
 
+/* Polynomials over Z/pZ on the host for the fraction-free rank below: a
+ * coefficient array and its length (no trailing zero coefficient). */
+typedef struct {
+    uint64_t *c;
+    slong len;
+} hpoly;
+
+static void hpoly_trim(hpoly *a)
+{
+    while (a->len > 0 && a->c[a->len - 1] == 0)
+        a->len--;
+}
+
+/* a*b - c*d over Z/pZ (schoolbook products) */
+static hpoly hpoly_cross(const hpoly *a, const hpoly *b, const hpoly *c, const hpoly *d, uint64_t p,
+                         const char *fn)
+{
+    const slong l1 = (a->len && b->len) ? a->len + b->len - 1 : 0;
+    const slong l2 = (c->len && d->len) ? c->len + d->len - 1 : 0;
+    hpoly r;
+    r.len = l1 > l2 ? l1 : l2;
+    r.c = xalloc((size_t)(r.len > 0 ? r.len : 1), fn);
+    memset(r.c, 0, (size_t)(r.len > 0 ? r.len : 1) * sizeof(uint64_t));
+    for (slong i = 0; i < a->len; i++)
+        for (slong j = 0; j < b->len; j++) {
+            const uint64_t t = mulmod_h(a->c[i], b->c[j], p);
+            r.c[i + j] = r.c[i + j] >= p - t ? r.c[i + j] - (p - t) : r.c[i + j] + t;
+        }
+    for (slong i = 0; i < c->len; i++)
+        for (slong j = 0; j < d->len; j++) {
+            const uint64_t t = mulmod_h(c->c[i], d->c[j], p);
+            r.c[i + j] = r.c[i + j] >= t ? r.c[i + j] - t : r.c[i + j] + (p - t);
+        }
+    hpoly_trim(&r);
+    return r;
+}
+
+/* a / b for b nonzero dividing a exactly (schoolbook long division) */
+static void hpoly_divexact(hpoly *a, const hpoly *b, uint64_t p, const char *fn)
+{
+    if (a->len == 0)
+        return;
+    if (a->len < b->len)
+        die(fn, "fraction-free rank: inexact division");
+    const uint64_t linv = powmod_h(b->c[b->len - 1], p - 2, p);
+    const slong ql = a->len - b->len + 1;
+    uint64_t *q = xalloc((size_t)ql, fn);
+    for (slong i = ql - 1; i >= 0; i--) {
+        const uint64_t f = mulmod_h(a->c[i + b->len - 1], linv, p);
+        q[i] = f;
+        for (slong j = 0; j < b->len; j++) {
+            const uint64_t t = mulmod_h(f, b->c[j], p);
+            uint64_t *x = &a->c[i + j];
+            *x = *x >= t ? *x - t : *x + (p - t);
+        }
+    }
+    for (slong i = 0; i < b->len - 1; i++)
+        if (a->c[i])
+            die(fn, "fraction-free rank: inexact division");
+    free(a->c);
+    a->c = q;
+    a->len = ql;
+    hpoly_trim(a);
+}
+
+/* Rank over Z/p(x) by fraction-free (Bareiss) elimination over Z/pZ[x]: after
+ * k pivots every remaining entry is a (k+1)-minor, divisible by the previous
+ * pivot, whatever columns were skipped.  Used where Z/pZ has too few points
+ * for the rank by evaluation (p <= min(r, c) deg M). */
+static slong polmat_rank_bareiss(const nmod_poly_mat_struct *M, const char *fn)
+{
+    const slong r = M->r, c = M->c;
+    const uint64_t p = M->modulus;
+    hpoly *A = (hpoly *)malloc((size_t)(r * c > 0 ? r * c : 1) * sizeof(hpoly));
+    if (!A)
+        die(fn, "out of host memory");
+    for (slong i = 0; i < r; i++)
+        for (slong j = 0; j < c; j++) {
+            const nmod_poly_struct *e = nmod_poly_mat_entry(M, i, j);
+            hpoly *h = &A[i * c + j];
+            h->len = e->length;
+            h->c = xalloc((size_t)(h->len > 0 ? h->len : 1), fn);
+            for (slong t = 0; t < h->len; t++)
+                h->c[t] = e->coeffs[t] % p;
+            hpoly_trim(h);
+        }
+    uint64_t one = 1;
+    hpoly prev = {&one, 1};
+    slong rank = 0;
+    for (slong col = 0; col < c && rank < r; col++) {
+        slong piv = -1;
+        for (slong i = rank; i < r && piv < 0; i++)
+            if (A[i * c + col].len)
+                piv = i;
+        if (piv < 0)
+            continue;
+        for (slong j = 0; j < c; j++) {
+            const hpoly t = A[piv * c + j];
+            A[piv * c + j] = A[rank * c + j];
+            A[rank * c + j] = t;
+        }
+        const hpoly *pv = &A[rank * c + col];
+        for (slong i = rank + 1; i < r; i++) {
+            for (slong j = col + 1; j < c; j++) {
+                hpoly x = hpoly_cross(pv, &A[i * c + j], &A[i * c + col], &A[rank * c + j], p, fn);
+                hpoly_divexact(&x, &prev, p, fn);
+                free(A[i * c + j].c);
+                A[i * c + j] = x;
+            }
+            A[i * c + col].len = 0;
+        }
+        prev = *pv;
+        rank++;
+    }
+    for (slong i = 0; i < r * c; i++)
+        free(A[i].c);
+    free(A);
+    return rank;
+}
+
 /* Rank over Z/p(x): a nonzero r x r minor has degree at most r*deg(M), so
  * it does not vanish at one of the points 0, 1, ..., r*deg(M); each point is
- * a device evaluation and a device elimination. */
+ * a device evaluation and a device elimination.  Moduli with fewer points
+ * take the fraction-free elimination over Z/pZ[x] on the host. */
 static slong polmat_rank(const nmod_poly_mat_struct *M, const char *fn)
 {
     const slong r = M->r, c = M->c;
@@ -1025,7 +1146,7 @@ static slong polmat_rank(const nmod_poly_mat_struct *M, const char *fn)
     const slong full = r < c ? r : c;
     const uint64_t p = M->modulus;
     if ((uint64_t)(full * d) >= p)
-        die(fn, "modulus too small for the exact rank by evaluation");
+        return polmat_rank_bareiss(M, fn);
     pmlg_ctx ctx = ctx_acquire(p, fn);
     const slong len = d + 1;
     uint64_t *P = pack(M, len, fn);

@@ -547,8 +547,8 @@ pmlg_status check_kernel_args(pmlg_ctx ctx, int64_t m, int64_t n, int64_t lenF, 
         return fail(PMLG_ERR_ARG, "kernel basis via approximation: at most 128 columns");
     if (!shift || !pivind || !nullity || !lenK)
         return fail(PMLG_ERR_ARG, "kernel basis: null output pointer");
-    if (!ctx->is_prime)
-        return fail(PMLG_ERR_UNSUPPORTED, "kernel bases need an odd prime modulus");
+    if (!ctx->is_prime && ctx->p != 2)
+        return fail(PMLG_ERR_UNSUPPORTED, "kernel bases need a prime modulus");
     return PMLG_OK;
 }
 

@@ -304,7 +304,7 @@ struct Mont32Arith {
     }
 };
 
-// Moduli from 2^62 to 2^64 (odd primes): plain residues, every product
+// Moduli from 2^62 to 2^64, and p = 2: plain residues, every product
 // reduced at once by the division of mod128_any (Moller-Granlund); sums of
 // residues may carry out of 64 bits, so additions go through addmod_any.
 // Slower than the Montgomery forms, which need p < 2^62 for their lazy sums.
@@ -1040,8 +1040,9 @@ pmlg_status leaf(pmlg_ctx ctx, Arena &ar, const uint64_t *F, int64_t strideF, in
     hipLaunchKernelGGL(leaf_init_kernel, dim3(grid_for((int64_t)m * m * cap + (int64_t)m * n * ordL)),
                        dim3(256), 0, s, a, F, strideF, lt);
     HIP_TRY(hipGetLastError(), "leaf_init_kernel");
-    const bool small = ctx->p < ((uint64_t)1 << 30);
-    const bool big = ctx->p >= ((uint64_t)1 << 62);
+    // p = 2 has no Montgomery form (p must be odd): plain residues as for p >= 2^62
+    const bool big = ctx->p >= ((uint64_t)1 << 62) || ctx->p == 2;
+    const bool small = !big && ctx->p < ((uint64_t)1 << 30);
     const size_t lds = step_lds_bytes(m, n, small ? 4 : 8);
     if (lds > kStepLdsMax)
         return fail(PMLG_ERR_UNSUPPORTED, "M-Basis: an m x n residual of this size does not fit "
@@ -1283,12 +1284,13 @@ pmlg_status basis_common(pmlg_ctx ctx, uint64_t *dP, int64_t capP, int64_t *lenP
 {
     if (m <= 0 || n <= 0 || m > kMaxM || n > kMaxN || order < 0 || lenF < 0)
         return fail(PMLG_ERR_ARG, "approximant basis: 1 <= m <= 256, 1 <= n <= 128, order >= 0");
-    if (step_lds_bytes((int)m, (int)n, ctx->p < ((uint64_t)1 << 30) ? 4 : 8) > kStepLdsMax)
+    if (step_lds_bytes((int)m, (int)n, ctx->p < ((uint64_t)1 << 30) && ctx->p != 2 ? 4 : 8) >
+        kStepLdsMax)
         return fail(PMLG_ERR_UNSUPPORTED,
                     "approximant basis: the m x n residual exceeds one workgroup's LDS (n (m | 1) "
                     "words plus the step's buffers: e.g. m = 256 allows n <= 62 for p >= 2^30)");
-    if (!ctx->is_prime)
-        return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
+    if (!ctx->is_prime && ctx->p != 2)
+        return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need a prime modulus");
     if (capP < order + 1)
         return fail(PMLG_ERR_ARG, "approximant basis: output capacity must be at least order + 1");
     int thres = ctx->leaf_order > 0 ? ctx->leaf_order : kPmbasisThres;
@@ -1458,8 +1460,8 @@ pmlg_status pmlg_pmbasis_leaf_order(pmlg_ctx ctx, int64_t m, int64_t n, int meth
         return st;
     if (m <= 0 || n <= 0 || m > kMaxM || n > kMaxN || !order)
         return fail(PMLG_ERR_ARG, "pmbasis leaf order: 1 <= m <= 256, 1 <= n <= 128");
-    if (!ctx->is_prime)
-        return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need an odd prime modulus");
+    if (!ctx->is_prime && ctx->p != 2)
+        return fail(PMLG_ERR_UNSUPPORTED, "approximant bases need a prime modulus");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (ctx->leaf_order > 0) {
         *order = ctx->leaf_order;

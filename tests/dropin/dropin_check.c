@@ -476,9 +476,12 @@ int main(void)
     check_basis(0, 5, 3, 60, 60, 1, P64);   /* 64-bit prime: plain-residue M-Basis steps */
     check_basis(1, 4, 2, 30, 30, 2, P63);
     check_basis(0, 160, 80, 40, 40, 1, P60); /* more than 128 rows */
+    check_basis(0, 5, 2, 120, 120, 1, 2);    /* p = 2 (a random 2-bit prime in the reference's tests) */
+    check_basis(1, 4, 3, 30, 30, 0, 3);
     check_kernel(6, 2, 8, 2, P60);
     check_kernel(7, 4, 5, 2, P30);
     check_kernel(5, 5, 6, 3, P30);
+    check_kernel(6, 2, 7, 1, 2);
     check_kernel(9, 3, 40, 3, P60);
     check_kernel(4, 6, 3, 1, P_NOFFT);
     check_kernel(6, 2, 8, 2, P64);

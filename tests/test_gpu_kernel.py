@@ -118,6 +118,23 @@ def test_kernel_shift_families(pml, algo, shift_name):
         check(pml, p, F, SHIFTS[shift_name](m, ln).astype(np.int64), algo)
 
 
+def test_kernel_prime_two(pml):
+    """Kernel bases modulo 2, both algorithms, bit-identical to the oracle."""
+    rng = np.random.default_rng(22)
+    for algo in ("zls", "via_approx"):
+        for (m, n, ln, kind) in [(6, 2, 9, "uniform"), (8, 3, 5, "rkdef"), (5, 4, 7, "rdeg")]:
+            F = family(rng, kind, m, n, ln, 2)
+            shift = rng.integers(0, 3, m).astype(np.int64)
+            # the oracle's is_kernel ranks by evaluation, which Z/2Z has too
+            # few points for: check K F = 0, degrees and pivots directly
+            nul, K = check(pml, 2, F, shift, algo, certify=False)
+            if nul:
+                assert not O.polmat_mul(K, F, 2).any()
+                _, _, piv, sh = ctx_for(pml, 2).kernel_basis(F, shift, algo)
+                assert np.array_equal(KO.row_degree(K, shift), sh)
+                assert np.array_equal(KO.pivot_index(K, shift)[0], piv)
+
+
 def test_kernel_row_degree_shift(pml):
     """shift = rdeg(F), the reference profile's default (p-kernel.c:75-77)."""
     rng = np.random.default_rng(3)

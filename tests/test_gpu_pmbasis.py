@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 P60 = 1152921504556515329
 P30 = 998244353
-PRIMES_ANY = [3, 5, 521, 524309, 536870923, 549755813911, 576460752303423619]  # order <= 32: leaf only
+PRIMES_ANY = [2, 3, 5, 521, 524309, 536870923, 549755813911, 576460752303423619]  # order <= 32: leaf only
 PRIMES_FFT = [P30, P60, 7340033, 469762049]
 
 
@@ -107,6 +107,18 @@ def test_leaf_any_prime(pml, p):
         sh = rng.integers(0, 3, m).astype(np.int64)
         check(pml, p, F, order, sh, "pmbasis")
         check(pml, p, F, order, sh, "mbasis")
+
+
+def test_pmbasis_prime_two(pml):
+    """p = 2 (the reference's tests can draw it: a random 2-bit prime):
+    plain-residue leaf arithmetic, products through the multi-prime path."""
+    rng = np.random.default_rng(2)
+    for (m, n, order, kind) in [(4, 2, 100, "uniform"), (7, 3, 200, "sparse"), (3, 1, 65, "uniform"),
+                                (6, 4, 130, "rkdef")]:
+        F = make(rng, kind, m, n, order, 2)
+        sh = rng.integers(0, 3, m).astype(np.int64)
+        P = check(pml, 2, F, order, sh)
+        assert O.is_approximant_basis(P, F, order, sh, 2) == 0
 
 
 def test_mbasis_long_order(pml):
