@@ -139,6 +139,10 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
         T.mem = mem;
         const uint64_t ninv = host_invmod((uint64_t)(N % p), p);
         const uint64_t ninvr = host_mulmod(ninv, ctx->mc.r64, p);
+        const uint64_t ninv2 = host_mulmod(ninv, 2 % p, p), ninvr2 = host_mulmod(ninvr, 2 % p, p);
+        // w^(N/4) and w^(N/4) / 2 (p odd: transforms need a prime p >= 3)
+        const uint64_t tquart = N >= 4 ? host_powmod(w, (uint64_t)(N / 4), p) : 0;
+        const uint64_t thalf = host_mulmod(tquart, (p + 1) / 2, p);
         auto fill = [&](auto *a, auto &tab, auto &col) {
             using Wt = std::remove_const_t<std::remove_pointer_t<decltype(a)>>;
             const Wt *full_tw = longN ? nullptr : a;
@@ -152,6 +156,14 @@ pmlg_status get_tables(pmlg_ctx ctx, int logN, const Tables **out)
             tab.sitw = a + stage0 + 2 * M;
             tab.ninvr = (Wt)ninvr;
             tab.ninvrq = (Wt)pre(ninvr);
+            tab.ninv2 = (Wt)ninv2;
+            tab.ninv2q = (Wt)pre(ninv2);
+            tab.ninvr2 = (Wt)ninvr2;
+            tab.ninvr2q = (Wt)pre(ninvr2);
+            tab.thalf = (Wt)thalf;
+            tab.thalfq = (Wt)pre(thalf);
+            tab.tquart = (Wt)tquart;
+            tab.tquartq = (Wt)pre(tquart);
             if (longN) {
                 col.rt = a + rt0;
                 col.rtq = a + rt0 + R / 2;
@@ -290,20 +302,23 @@ pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, v
     const int64_t N = (int64_t)1 << logN;
     const W p = (W)ctx->p;
     const int logP = eval_log_run_in<W>(logN), logPc = eval_log_run_out<W>(logN);
+    // a product of length <= 3N/4 (no wrap-around) needs 3N/4 evaluations only
+    const bool trunc = trunc_ok<W>(logN, la + lb - 1) && hi <= 3 * (N >> 2);
+    const int64_t npts = trunc ? 3 * (N >> 2) : N;
     const NttTune &tune = ctx->tune;
     Prof pr;
     pr.stage = -1;
     if (prof)
         prof_begin(ctx, 0, s, &pr);
     HIP_TRY((launch_ntt_fwd<uint64_t, W>(dA, strideA, la, Ah, m * k, logN, logP, tune, tab, p, s,
-                                         tmpf, ct)),
+                                         tmpf, ct, trunc)),
             "ntt_fwd(A)");
     if (prof) {
         prof_end(ctx, s, &pr);
         prof_begin(ctx, 0, s, &pr);
     }
     HIP_TRY((launch_ntt_fwd<uint64_t, W>(dB, strideB, lb, Bh, k * n, logN, logP, tune, tab, p, s,
-                                         tmpf, ct)),
+                                         tmpf, ct, trunc)),
             "ntt_fwd(B)");
     if (prof)
         prof_end(ctx, s, &pr);
@@ -316,17 +331,19 @@ pmlg_status run_product_on(pmlg_ctx ctx, const Tables *T, W *Ah, W *Bh, W *Ch, v
         mc.redc_small = vmax < (((unsigned __int128)(3 * ctx->p - 1)) << 64) ? 1 : 0;
         itab.ninv = tab.ninvr;
         itab.ninvq = tab.ninvrq;
+        itab.ninv2 = tab.ninvr2;
+        itab.ninv2q = tab.ninvr2q;
     }
     if (prof)
         prof_begin(ctx, 1, s, &pr);
-    HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, N, logP, logPc, mc, s),
+    HIP_TRY(launch_pointwise<W>(kind, Ah, Bh, Ch, (int)m, (int)k, (int)n, npts, logP, logPc, mc, s),
             "pointwise");
     if (prof) {
         prof_end(ctx, s, &pr);
         prof_begin(ctx, 2, s, &pr);
     }
     HIP_TRY((launch_ntt_inv<W, uint64_t>(Ch, dOut, out_stride, lo, hi, m * n, logN, logPc, tune,
-                                         itab, p, tmp, s, ct)),
+                                         itab, p, tmp, s, ct, trunc)),
             "ntt_inv");
     if (prof)
         prof_end(ctx, s, &pr);

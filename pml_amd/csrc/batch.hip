@@ -289,6 +289,7 @@ struct FwdGroup {
     size_t seg_off;  // byte offset of the segment array in the plan's table block
     int nseg;
     int64_t blocks;
+    bool trunc;  // 3/4-truncated products (their own launches: the split may differ)
 };
 struct InvGroup {
     pmlg_ctx ctx;
@@ -297,6 +298,7 @@ struct InvGroup {
     size_t seg_off;
     int nseg;
     int64_t blocks, mblocks;
+    bool trunc;
 };
 struct PwGroup {
     pmlg_ctx ctx;
@@ -390,7 +392,7 @@ pmlg_status launch_groups(pmlg_batch_plan plan, hipStream_t s)
             return st;
         HIP_TRY((launch_ntt_fwd_segs<uint64_t, W>(
                     reinterpret_cast<const FwdSeg<uint64_t, W> *>(tb + g.seg_off), g.nseg,
-                    g.blocks, g.logN, g.logP, g.fwd_e, *tab, (W)g.ctx->p, s)),
+                    g.blocks, g.logN, g.logP, g.fwd_e, *tab, (W)g.ctx->p, s, g.trunc)),
                 "batched forward transforms");
     }
     return PMLG_OK;
@@ -428,10 +430,12 @@ pmlg_status launch_inv(pmlg_batch_plan plan, hipStream_t s)
         if (g.redc) {
             itab.ninv = tab->ninvr;
             itab.ninvq = tab->ninvrq;
+            itab.ninv2 = tab->ninvr2;
+            itab.ninv2q = tab->ninvr2q;
         }
         HIP_TRY((launch_ntt_inv_segs<W, uint64_t>(
                     reinterpret_cast<const InvSeg<W, uint64_t> *>(tb + g.seg_off), g.nseg,
-                    g.blocks, g.mblocks, g.logN, g.logP, g.inv_e, itab, (W)g.ctx->p, s)),
+                    g.blocks, g.mblocks, g.logN, g.logP, g.inv_e, itab, (W)g.ctx->p, s, g.trunc)),
                 "batched inverse transforms");
     }
     return PMLG_OK;
@@ -489,7 +493,7 @@ inverse:
 
 template <typename W>
 void add_segments(const pmlg_mul_desc &d, const ProdInfo &pi, unsigned char *arena,
-                  std::map<std::tuple<pmlg_ctx, int, int>, std::vector<FwdSeg<uint64_t, W>>> &fwd,
+                  std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<FwdSeg<uint64_t, W>>> &fwd,
                   std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<InvSeg<W, uint64_t>>> &inv,
                   std::map<std::tuple<pmlg_ctx, int, int>, std::vector<PwSeg<W>>> &pw,
                   std::map<std::tuple<pmlg_ctx, int, int>, size_t> &pw_lds)
@@ -503,13 +507,15 @@ void add_segments(const pmlg_mul_desc &d, const ProdInfo &pi, unsigned char *are
     W *Bh = reinterpret_cast<W *>(arena) + pi.bh;
     W *Ch = reinterpret_cast<W *>(arena) + pi.ch;
     W *tmp = pi.tmp != (size_t)-1 ? reinterpret_cast<W *>(arena) + pi.tmp : nullptr;
-    auto &fv = fwd[{ctx, logN, cf.fwd_e}];
-    fv.push_back(FwdSeg<uint64_t, W>{d.A, Ah, d.lenA, d.lenA, d.m * d.k, 0});
-    fv.push_back(FwdSeg<uint64_t, W>{d.B, Bh, d.lenB, d.lenB, d.k * d.n, 0});
-    const bool redc = cf.kind == PW_MFMA && sizeof(W) == 8;
     const int64_t lenC = d.lenA + d.lenB - 1;
-    inv[{ctx, logN, cf.inv_e, redc ? 1 : 0}].push_back(
-        InvSeg<W, uint64_t>{Ch, d.C, tmp, lenC, 0, lenC, d.m * d.n, 0, 0});
+    // a product of length <= 3N/4 needs 3N/4 evaluations only
+    const int trunc = trunc_ok<W>(logN, lenC) ? 1 : 0;
+    auto &fv = fwd[{ctx, logN, cf.fwd_e, trunc}];
+    fv.push_back(FwdSeg<uint64_t, W>{d.A, Ah, d.lenA, d.lenA, d.m * d.k, 0, trunc, 0});
+    fv.push_back(FwdSeg<uint64_t, W>{d.B, Bh, d.lenB, d.lenB, d.k * d.n, 0, trunc, 0});
+    const bool redc = cf.kind == PW_MFMA && sizeof(W) == 8;
+    inv[{ctx, logN, cf.inv_e, (redc ? 1 : 0) | (trunc << 1)}].push_back(
+        InvSeg<W, uint64_t>{Ch, d.C, tmp, lenC, 0, lenC, d.m * d.n, 0, 0, trunc, 0});
     int pb = 0;
     if (cf.kind == PW_MFMA) {
         pb = pw_pick_pb((int)d.m, (int)d.k, (int)d.n, ctx->mc.limbs, logP) >> cf.pb_shift;
@@ -524,7 +530,7 @@ void add_segments(const pmlg_mul_desc &d, const ProdInfo &pi, unsigned char *are
     ps.n = (int32_t)d.n;
     ps.logP = logP;
     ps.logPc = eval_log_run_out<W>(logN);
-    ps.N = N;
+    ps.N = trunc ? 3 * (N >> 2) : N;
     ps.redc_small = 0;
     if (redc) {
         const unsigned __int128 vmax = (unsigned __int128)d.k * (ctx->p - 1) * (ctx->p - 1);
@@ -540,34 +546,36 @@ void add_segments(const pmlg_mul_desc &d, const ProdInfo &pi, unsigned char *are
 
 template <typename W>
 void finish_groups(pmlg_batch_plan plan, Blob &blob,
-                   std::map<std::tuple<pmlg_ctx, int, int>, std::vector<FwdSeg<uint64_t, W>>> &fwd,
+                   std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<FwdSeg<uint64_t, W>>> &fwd,
                    std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<InvSeg<W, uint64_t>>> &inv,
                    std::map<std::tuple<pmlg_ctx, int, int>, std::vector<PwSeg<W>>> &pw,
                    std::map<std::tuple<pmlg_ctx, int, int>, size_t> &pw_lds)
 {
     for (auto &kv : fwd) {
         const int logN = std::get<1>(kv.first);
+        const bool trunc = std::get<3>(kv.first) != 0;
         int64_t b = 0;
         for (auto &sg : kv.second) {
             sg.block0 = b;
-            b += ntt_seg_blocks<W>(sg.count, logN);
+            b += ntt_seg_blocks<W>(sg.count, logN, trunc);
         }
         FwdGroup g{std::get<0>(kv.first), logN, eval_log_run_in<W>(logN),
-                   std::get<2>(kv.first), blob.add(kv.second), (int)kv.second.size(), b};
+                   std::get<2>(kv.first), blob.add(kv.second), (int)kv.second.size(), b, trunc};
         plan->fwd.push_back(g);
     }
     for (auto &kv : inv) {
         const int logN = std::get<1>(kv.first);
+        const bool trunc = (std::get<3>(kv.first) & 2) != 0;
         int64_t b = 0, mb = 0;
         for (auto &sg : kv.second) {
             sg.block0 = b;
             sg.mblock0 = mb;
-            b += ntt_seg_blocks<W>(sg.count, logN);
-            mb += merge_seg_blocks<W>(sg.count, sg.hi - sg.lo, logN);
+            b += ntt_seg_blocks<W>(sg.count, logN, trunc);
+            mb += merge_seg_blocks<W>(sg.count, sg.hi - sg.lo, logN, trunc);
         }
         InvGroup g{std::get<0>(kv.first), logN, eval_log_run_out<W>(logN),
-                   std::get<2>(kv.first), std::get<3>(kv.first) != 0, blob.add(kv.second),
-                   (int)kv.second.size(), b, mb};
+                   std::get<2>(kv.first), (std::get<3>(kv.first) & 1) != 0, blob.add(kv.second),
+                   (int)kv.second.size(), b, mb, trunc};
         plan->inv.push_back(g);
     }
     for (auto &kv : pw) {
@@ -722,8 +730,8 @@ pmlg_status build_plan(pmlg_batch_plan *out, const pmlg_mul_desc *descs, int64_t
     plan->arena_bytes = arena_bytes;
 
     Blob blob;
-    std::map<std::tuple<pmlg_ctx, int, int>, std::vector<FwdSeg<uint64_t, uint64_t>>> f64;
-    std::map<std::tuple<pmlg_ctx, int, int>, std::vector<FwdSeg<uint64_t, uint32_t>>> f32;
+    std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<FwdSeg<uint64_t, uint64_t>>> f64;
+    std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<FwdSeg<uint64_t, uint32_t>>> f32;
     std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<InvSeg<uint64_t, uint64_t>>> i64;
     std::map<std::tuple<pmlg_ctx, int, int, int>, std::vector<InvSeg<uint32_t, uint64_t>>> i32;
     std::map<std::tuple<pmlg_ctx, int, int>, std::vector<PwSeg<uint64_t>>> p64;

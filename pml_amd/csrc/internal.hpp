@@ -19,6 +19,9 @@ struct NttTab {
     // x < 2^(logM-1-s), stored interleaved from offset 2*(M - 2^(logM-s)).
     const W *stw, *sitw;
     W ninvr, ninvrq;  // N^-1 * 2^64 mod p: scaling after a Montgomery (REDC) pointwise product
+    // 3/4-truncated inverses: 2 N^-1 (2 N^-1 2^64 after REDC), w^(N/4) / 2
+    // and w^(N/4)
+    W ninv2, ninv2q, ninvr2, ninvr2q, thalf, thalfq, tquart, tquartq;
 };
 
 // Tables of the long transforms (N = R M with R > 4): the column pass runs
@@ -84,25 +87,41 @@ inline int eval_log_run_out(int logN)
     return eval_log_run_in<W>(logN);
 }
 
+// 3/4-truncated transforms: a product of length <= 3N/4 needs only the
+// evaluations at the first 3N/4 positions (the frequencies 0, 1 and 2 mod 4:
+// the DIF output order of one sub-transform, or the first three of four
+// sub-transforms when N exceeds one, split in four), for kTruncMinLog <=
+// logN <= max_log_m + 2 (3N/4 then holds whole point blocks of 8).
+constexpr int kTruncMinLog = 5;
+
+template <typename W>
+inline bool trunc_ok(int logN, int64_t lenC)
+{
+    return logN >= kTruncMinLog && logN <= NttLimits<W>::max_log_m + 2 &&
+           lenC <= 3 * (((int64_t)1 << logN) >> 2);
+}
+
 // Forward transforms of `count` entries: entry e reads in[e*in_stride + i]
-// for i < in_len (zero beyond), writes its N evaluations in blocked layout.
-// Transforms longer than 4 times the LDS limit run a column pass first into
-// tmp (count*N words of Win) with the tables ct.
+// for i < in_len (zero beyond), writes its N evaluations in blocked layout
+// (the first 3N/4 when trunc).  Transforms longer than 4 times the LDS limit
+// run a column pass first into tmp (count*N words of Win) with the tables ct.
 template <typename Win, typename W>
 hipError_t launch_ntt_fwd(const Win *in, int64_t in_stride, int64_t in_len, W *out,
                           int64_t count, int logN, int logP, const NttTune &tune,
                           const NttTab<W> &tab, W p, hipStream_t s, Win *tmp = nullptr,
-                          const ColTab<W> *ct = nullptr);
+                          const ColTab<W> *ct = nullptr, bool trunc = false);
 
 // Inverse transforms (blocked input): coefficients [lo, hi) of each entry
 // (clipped to N), scaled by 1/N, written to out[e*out_stride + (i - lo)] as
 // Wout in [0,p).  tmp must hold count*N words of W when N exceeds the LDS
-// limit; longer than 4 times that limit, the column pass uses ct.
+// limit; longer than 4 times that limit, the column pass uses ct.  trunc:
+// the input holds the first 3N/4 evaluations of a polynomial of length
+// <= 3N/4 (hi <= 3N/4).
 template <typename W, typename Wout>
 hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
                           int64_t count, int logN, int logP, const NttTune &tune,
                           const NttTab<W> &tab, W p, W *tmp, hipStream_t s,
-                          const ColTab<W> *ct = nullptr);
+                          const ColTab<W> *ct = nullptr, bool trunc = false);
 
 // ---------------------------------------------------------------------------
 // Segment tables of the batched launches.  One launch serves many operand
@@ -118,6 +137,7 @@ struct FwdSeg {
     const Win *in;
     W *out;
     int64_t in_stride, in_len, count, block0;
+    int32_t trunc, pad;  // trunc: evaluations at the first 3N/4 positions only
 };
 
 // Inverse: `count` entries of blocked evaluations at in; coefficients [lo, hi)
@@ -129,23 +149,26 @@ struct InvSeg {
     Wout *out;
     W *tmp;
     int64_t out_stride, lo, hi, count, block0, mblock0;
+    int32_t trunc, pad;  // trunc: 3N/4 evaluations of a product of length <= 3N/4
 };
 
 // Workgroups one segment of `count` entries needs at length 2^logN (transform
 // launch), and for the merge pass (0 when N fits in LDS).
 template <typename W>
-int64_t ntt_seg_blocks(int64_t count, int logN);
+int64_t ntt_seg_blocks(int64_t count, int logN, bool trunc);
 template <typename W>
-int64_t merge_seg_blocks(int64_t count, int64_t width, int logN);
+int64_t merge_seg_blocks(int64_t count, int64_t width, int logN, bool trunc);
 
 // segs: device array of nseg segments; blocks / mblocks: totals of the above.
+// trunc: every segment of the launch is 3/4-truncated (their trunc flags set).
 template <typename Win, typename W>
 hipError_t launch_ntt_fwd_segs(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logN,
-                               int logP, int log_e, const NttTab<W> &tab, W p, hipStream_t s);
+                               int logP, int log_e, const NttTab<W> &tab, W p, hipStream_t s,
+                               bool trunc);
 template <typename W, typename Wout>
 hipError_t launch_ntt_inv_segs(const InvSeg<W, Wout> *segs, int nseg, int64_t blocks,
                                int64_t mblocks, int logN, int logP, int log_e,
-                               const NttTab<W> &tab, W p, hipStream_t s);
+                               const NttTab<W> &tab, W p, hipStream_t s, bool trunc);
 
 enum PointwiseKind { PW_PLAIN = 0, PW_MFMA = 1 };
 

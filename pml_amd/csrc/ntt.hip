@@ -21,6 +21,11 @@ constexpr int kThreads = 256;
 #define PMLG_INV_WAVES 4
 #endif
 constexpr int kInvWaves = PMLG_INV_WAVES;  // minimum waves per SIMD of the inverse register kernels
+// minimum waves per SIMD of the radix-32 forward register kernels: their
+// 256-thread workgroups alone ask for one, which lets the register allocator
+// drift past 256 registers (one wave per SIMD, 1.6x slower) on small changes
+// of the surrounding code; two keep them at or below 256.
+constexpr int kFwdWaves32 = 2;
 
 // Point-blocked evaluation layout: position `pos` of entry `e` (of `count`
 // entries) lives at ((pos >> logP) * count + e) * P + (pos mod P), P = 2^logP,
@@ -45,13 +50,64 @@ __device__ __forceinline__ W tw_mul(W x, const W *w, const W *wq, int64_t idx, W
     return shoup_lazy(x, w[idx], wq[idx], p);
 }
 
+// The last two inverse stages of a 3/4-truncated product.  The forward's
+// first two stages map the coefficients x0 = c[i], x1 = c[i + N/4], x2 =
+// c[i + N/2], x3 = c[i + 3N/4] (i < N/4) to Y0 = x0 + x1 + x2 + x3, Y1 = (x0 +
+// x2 - x1 - x3) w^(2i), Y2 = w^i ((x0 - x2) + s (x1 - x3)), Y3 (dropped),
+// s = w^(N/4).  A product of length <= 3N/4 has x3 = 0, so with the inverse's
+// earlier stages leaving (N/4) Y0, (N/4) Y1, (N/4) Y2, the coefficients are
+//   x1 = 2/N (Y0 - u),  x0 = S + D - (s/2) x1,  x2 = S - D + (s/2) x1,
+// u = w^(-2i) Y1, S = (Y0 + u)/N, D = 2/N w^(-i) Y2, scaled like the other
+// outputs (N^-1, or N^-1 2^64 after a REDC pointwise product).  Any word is
+// a valid input (each is first taken through a Shoup product); the outputs
+// are in [0, p).
+template <typename W>
+__device__ __forceinline__ void trunc_solve(W &y0, W &y1, W &y2, int64_t i, const NttTab<W> &tab,
+                                            W p)
+{
+    const W a = reduce_2p<W>(shoup_lazy(y0, tab.itw[0], tab.itwq[0], p), p);
+    const W u = reduce_2p<W>(shoup_lazy(y1, tab.itw[2 * i], tab.itwq[2 * i], p), p);
+    const W v = reduce_2p<W>(shoup_lazy(y2, tab.itw[i], tab.itwq[i], p), p);
+    const W d = a >= u ? a - u : a + (p - u);
+    W su = a + u;
+    su = su >= p ? su - p : su;
+    const W x1 = shoup<W>(d, tab.ninv2, tab.ninv2q, p);
+    const W S = shoup<W>(su, tab.ninv, tab.ninvq, p);
+    const W D = shoup<W>(v, tab.ninv2, tab.ninv2q, p);
+    const W h = shoup<W>(x1, tab.thalf, tab.thalfq, p);
+    W sd = S + D;
+    sd = sd >= p ? sd - p : sd;
+    W md = S >= D ? S - D : S + (p - D);
+    y0 = sd >= h ? sd - h : sd + (p - h);
+    md = md + h;
+    y2 = md >= p ? md - p : md;
+    y1 = x1;
+}
+
+// Output correction of a truncated product inverted as a whole: with any
+// values at the dropped evaluations (frequencies 3 mod 4, contributing U3'
+// where U3 was due), the scaled outputs are c_t' = c_t + s^(-3t) (U3' - U3)/4
+// for t = 0..3 (c_t = c[i + tN/4], s = w^(N/4), s^2 = -1), and c_3 = 0 makes
+// (U3' - U3)/4 = s c_3'.  So c_0 = c_0' - s c_3', c_1 = c_1' + c_3', c_2 =
+// c_2' + s c_3', from scaled outputs in [0, p).
+template <typename W>
+__device__ __forceinline__ void trunc_correct(W &c0, W &c1, W &c2, W c3, const NttTab<W> &tab, W p)
+{
+    const W sc = shoup<W>(c3, tab.tquart, tab.tquartq, p);
+    c0 = c0 >= sc ? c0 - sc : c0 + (p - sc);
+    c1 = c1 + c3;
+    c1 = c1 >= p ? c1 - p : c1;
+    c2 = c2 + sc;
+    c2 = c2 >= p ? c2 - p : c2;
+}
+
 // Units u0 .. u0+nb-1 (of `units` = count << logR) of one operand: the
 // workgroup body shared by the direct and the segment-table launches.
 template <typename Win, typename W>
 __device__ __forceinline__ void fwd_lds_body(const Win *__restrict__ in, int64_t in_stride,
                                              int64_t in_len, W *__restrict__ out, int64_t units,
                                              int64_t u0, int logN, int logR, int lognb, int logP,
-                                             const NttTab<W> &tab, W p)
+                                             const NttTab<W> &tab, W p, bool trunc = false)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *s = reinterpret_cast<W *>(smem_raw);
@@ -108,12 +164,13 @@ __device__ __forceinline__ void fwd_lds_body(const Win *__restrict__ in, int64_t
         __syncthreads();
     }
 
-    // store: unit u = (e, g) -> out[e*N + g*M + i]
+    // store: unit u = (e, g) -> out[e*N + g*M + i]; a 3/4-truncated transform
+    // (R = 1) keeps the positions below 3M/4
     for (int idx = threadIdx.x; idx < nb * M; idx += kThreads) {
         const int t = idx >> logM;
         const int i1 = idx & (M - 1);
         const int64_t u = u0 + t;
-        if (u < units) {
+        if (u < units && !(trunc && R == 1 && i1 >= 3 * (M >> 2))) {
             const int64_t e = u >> logR;
             const int g = (int)(u & (R - 1));
             out[blk_addr(e, (int64_t)g * M + i1, units >> logR, logP)] = reduce_2p<W>(s[idx], p);
@@ -125,10 +182,10 @@ template <typename Win, typename W>
 __global__ void __launch_bounds__(kThreads)
 ntt_fwd_kernel(const Win *__restrict__ in, int64_t in_stride, int64_t in_len,
                W *__restrict__ out, int64_t units, int logN, int logR, int lognb, int logP,
-               NttTab<W> tab, W p)
+               NttTab<W> tab, W p, int trunc)
 {
     fwd_lds_body<Win, W>(in, in_stride, in_len, out, units, (int64_t)blockIdx.x << lognb, logN,
-                         logR, lognb, logP, tab, p);
+                         logR, lognb, logP, tab, p, trunc != 0);
 }
 
 template <typename W, typename Wout>
@@ -136,7 +193,7 @@ __device__ __forceinline__ void inv_lds_body(const W *__restrict__ in, Wout *__r
                                              int64_t out_stride, int64_t lo, int64_t hi,
                                              W *__restrict__ tmp, int64_t units, int64_t u0,
                                              int logN, int logR, int lognb, int logP,
-                                             const NttTab<W> &tab, W p)
+                                             const NttTab<W> &tab, W p, bool trunc = false)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *s = reinterpret_cast<W *>(smem_raw);
@@ -155,8 +212,10 @@ __device__ __forceinline__ void inv_lds_body(const W *__restrict__ in, Wout *__r
         if (u < units) {
             const int64_t e = u >> logR;
             const int g = (int)(u & (R - 1));
-            v = in[blk_addr(e, (int64_t)g * M + i1, units >> logR, logP)];
-            v = v >= p2 ? v - p2 : v;  // products may arrive in [0, 4p)
+            if (!(trunc && i1 >= 3 * (M >> 2))) {  // a truncated product's dropped quarter
+                v = in[blk_addr(e, (int64_t)g * M + i1, units >> logR, logP)];
+                v = v >= p2 ? v - p2 : v;  // products may arrive in [0, 4p)
+            }
         }
         s[idx] = v;
     }
@@ -181,6 +240,22 @@ __device__ __forceinline__ void inv_lds_body(const W *__restrict__ in, Wout *__r
         __syncthreads();
     }
 
+    if (trunc && R == 1) {
+        // scaled outputs, corrected by the dropped quarter (trunc_correct)
+        const int M4 = M >> 2;
+        for (int b = threadIdx.x; b < nb * M4; b += kThreads) {
+            const int t = b >> (logM - 2);
+            const int i = b & (M4 - 1);
+            W *q = s + t * M + i;
+            W c0 = shoup<W>(q[0], tab.ninv, tab.ninvq, p), c1 = shoup<W>(q[M4], tab.ninv, tab.ninvq, p);
+            W c2 = shoup<W>(q[2 * M4], tab.ninv, tab.ninvq, p);
+            trunc_correct<W>(c0, c1, c2, shoup<W>(q[3 * M4], tab.ninv, tab.ninvq, p), tab, p);
+            q[0] = c0;
+            q[M4] = c1;
+            q[2 * M4] = c2;
+        }
+        __syncthreads();
+    }
     for (int idx = threadIdx.x; idx < nb * M; idx += kThreads) {
         const int t = idx >> logM;
         const int i1 = idx & (M - 1);
@@ -191,7 +266,7 @@ __device__ __forceinline__ void inv_lds_body(const W *__restrict__ in, Wout *__r
         if (R == 1) {
             if (i1 >= lo && i1 < hi)
                 out[e * out_stride + (i1 - lo)] =
-                    (Wout)shoup<W>(s[idx], tab.ninv, tab.ninvq, p);
+                    (Wout)(trunc && R == 1 ? s[idx] : shoup<W>(s[idx], tab.ninv, tab.ninvq, p));
         } else {
             const int g = (int)(u & (R - 1));
             tmp[e * N + (int64_t)g * M + i1] = s[idx];
@@ -203,10 +278,10 @@ template <typename W, typename Wout>
 __global__ void __launch_bounds__(kThreads)
 ntt_inv_kernel(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
                int64_t hi, W *__restrict__ tmp, int64_t units, int logN, int logR, int lognb,
-               int logP, NttTab<W> tab, W p)
+               int logP, NttTab<W> tab, W p, int trunc)
 {
     inv_lds_body<W, Wout>(in, out, out_stride, lo, hi, tmp, units, (int64_t)blockIdx.x << lognb,
-                          logN, logR, lognb, logP, tab, p);
+                          logN, logR, lognb, logP, tab, p, trunc != 0);
 }
 
 // x[o] = N^-1 * sum_g w^(-(o g) mod N) z_g[o mod M], o in [lo, hi)
@@ -232,6 +307,41 @@ __device__ __forceinline__ void merge_one(const W *__restrict__ z, Wout *__restr
         acc = add_2p<W>(acc, tw_mul<W>(ze[(int64_t)g * M + i1], tab.itw, tab.itwq, widx, p), p2);
     }
     out[e * out_stride + (o - lo)] = (Wout)shoup<W>(acc, tab.ninv, tab.ninvq, p);
+}
+
+// Merge of a truncated product split in four (N = 4M, the sub-transform of
+// frequencies 3 mod 4 dropped): item (e, i), i < M, turns z_0[i], z_1[i],
+// z_2[i] (the sub-inverses, z_g = M w^(i g) sum_t c[i + tM] w^(tMg)) into
+// c[i], c[i + M], c[i + 2M] (c[i + 3M] = 0): the relations of trunc_solve
+// with z_2 in the role of the frequencies 2 mod 4 and z_1 of 1 mod 4.
+template <typename W, typename Wout>
+__device__ __forceinline__ void merge_trunc_one(const W *__restrict__ z, Wout *__restrict__ out,
+                                                int64_t out_stride, int64_t lo, int64_t hi,
+                                                int64_t idx, int logN, const NttTab<W> &tab, W p)
+{
+    const int64_t N = (int64_t)1 << logN, M = N >> 2;
+    const int64_t e = idx / M, i = idx - e * M;
+    const W *ze = z + e * N;
+    W c0 = ze[i], c1 = ze[2 * M + i], c2 = ze[M + i];
+    trunc_solve<W>(c0, c1, c2, i, tab, p);
+    Wout *o = out + e * out_stride;
+    if (i >= lo && i < hi)
+        o[i - lo] = (Wout)c0;
+    if (i + M >= lo && i + M < hi)
+        o[i + M - lo] = (Wout)c1;
+    if (i + 2 * M >= lo && i + 2 * M < hi)
+        o[i + 2 * M - lo] = (Wout)c2;
+}
+
+template <typename W, typename Wout>
+__global__ void __launch_bounds__(kThreads)
+ntt_merge_trunc_kernel(const W *__restrict__ z, Wout *__restrict__ out, int64_t out_stride,
+                       int64_t lo, int64_t hi, int64_t count, int logN, NttTab<W> tab, W p)
+{
+    const int64_t total = count << (logN - 2);
+    for (int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * kThreads)
+        merge_trunc_one<W, Wout>(z, out, out_stride, lo, hi, idx, logN, tab, p);
 }
 
 template <typename W, typename Wout>
@@ -658,7 +768,7 @@ __device__ __forceinline__ void inv_stages(W (&x)[1 << LOGE], const W (&tw)[2 * 
 template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0>
 __device__ __forceinline__ void fwd_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 << LOGE) - 1)],
                                            W *lds, int t, const W *__restrict__ stw,
-                                           const Bfly<W> &k, bool half)
+                                           const Bfly<W> &k, bool half, bool trunc)
 {
     using PS = PassShape<L, LOGE, S0>;
     if constexpr (S0 == 0) {
@@ -669,7 +779,16 @@ __device__ __forceinline__ void fwd_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 <<
         if constexpr (PS::R > 1)
             fwd_stages<W, LOOSE, L, LOGE, 0, B0, PS::R - 2>(x, tw, k);
     } else {
-        fwd_stages<W, LOOSE, L, LOGE, S0, B0, PS::R - 1>(x, tw, k);
+        // later passes act below the index's top two bits, which come from
+        // the thread (one group per thread): a thread whose positions all
+        // lie in the dropped quarter (whole waves for most shapes) skips.
+        // Not with 16 words per thread: the branch takes the kernel past
+        // the 128 registers of four waves per SIMD.
+        bool skip = false;
+        if constexpr (PS::G == 1 && LOGE >= 5)
+            skip = trunc && PS::index(t, 0, 0) >= (3 << (L - 2));
+        if (!skip)
+            fwd_stages<W, LOOSE, L, LOGE, S0, B0, PS::R - 1>(x, tw, k);
     }
     if constexpr (S0 + PS::R < L) {
         load_pass_tw<W, L, LOGE, S0 + PS::R>(tw, t, stw);  // in flight during the exchange
@@ -678,7 +797,7 @@ __device__ __forceinline__ void fwd_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 <<
         lds_load<W, L, LOGE, S0 + PS::R>(x, lds, t);
         __syncthreads();
         constexpr int B1 = LOOSE ? fwd_pass_out(PS::R, B0, false) : 2;
-        fwd_passes<W, LOOSE, L, LOGE, S0 + PS::R, B1>(x, tw, lds, t, stw, k, false);
+        fwd_passes<W, LOOSE, L, LOGE, S0 + PS::R, B1>(x, tw, lds, t, stw, k, false, trunc);
     }
 }
 
@@ -768,21 +887,29 @@ __device__ __forceinline__ void load_words(const W *src, W *v)
 // the pass at S0); on return x holds the pass-0 arrangement, below
 // inv_pass_out(...) p in lazy mode (any 64-bit value is a valid input of the
 // final scaling or of the merge pass's twist)
+// trunc (3/4-truncated product): passes above the last skip the threads
+// whose positions all lie in the dropped quarter (their values only reach
+// that quarter's outputs, which the caller's correction uses whatever they
+// are).
 template <typename W, bool LOOSE, int L, int LOGE, int S0, int B0>
 __device__ __forceinline__ void inv_passes(W (&x)[1 << LOGE], W (&tw)[2 * ((1 << LOGE) - 1)],
-                                           W *lds, int t, const W *__restrict__ sitw,
-                                           const Bfly<W> &k)
+                                           W *lds, int t, const NttTab<W> &tab,
+                                           const Bfly<W> &k, bool trunc)
 {
     using PS = PassShape<L, LOGE, S0>;
-    inv_stages<W, LOOSE, L, LOGE, S0, B0, 0>(x, tw, k);
+    bool skip = false;
+    if constexpr (S0 > 0 && PS::G == 1 && LOGE >= 5)  // (see fwd_passes)
+        skip = trunc && PS::index(t, 0, 0) >= (3 << (L - 2));
+    if (!skip)
+        inv_stages<W, LOOSE, L, LOGE, S0, B0, 0>(x, tw, k);
     if constexpr (S0 > 0) {
-        load_pass_tw<W, L, LOGE, S0 - LOGE>(tw, t, sitw);
+        load_pass_tw<W, L, LOGE, S0 - LOGE>(tw, t, tab.sitw);
         __syncthreads();  // every thread has finished reading the previous arrangement
         lds_store<W, L, LOGE, S0>(x, lds, t);
         __syncthreads();
         lds_load<W, L, LOGE, S0 - LOGE>(x, lds, t);
         constexpr int B1 = LOOSE ? inv_pass_out(PS::R, B0, PS::BLO == 0) : 2;
-        inv_passes<W, LOOSE, L, LOGE, S0 - LOGE, B1>(x, tw, lds, t, sitw, k);
+        inv_passes<W, LOOSE, L, LOGE, S0 - LOGE, B1>(x, tw, lds, t, tab, k, trunc);
     }
 }
 
@@ -792,7 +919,7 @@ template <typename Win, typename W, bool LOOSE, int L, int LOGE>
 __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t in_stride,
                                              int64_t in_len, W *__restrict__ out, int64_t units,
                                              int64_t u, int logR, int logP, const NttTab<W> &tab,
-                                             W p, bool pre = false)
+                                             W p, bool pre = false, bool trunc = false)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
@@ -805,6 +932,12 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
     const int64_t N = (int64_t)1 << logN;
     const W p2 = p + p;
     const Win *xin = in + e * in_stride;
+    // truncated: with a radix-4 split the fourth sub-transform (frequencies
+    // 3 mod 4) is dropped (the whole workgroup returns), without a split the
+    // last quarter of the positions
+    if (trunc && R == 4 && gg == 3)
+        return;
+    const bool tq = trunc && R == 1;
 
     W tw[2 * (E - 1)];
     load_pass_tw<W, L, LOGE, 0>(tw, t, tab.stw);
@@ -838,7 +971,7 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
     // pass's reduced rows); a direct load of at most M/2 coefficients leaves
     // the upper half zero
     fwd_passes<W, LOOSE, L, LOGE, 0, 2>(x, tw, lds, t, tab.stw, k,
-                                        !pre && R == 1 && in_len <= (M >> 1));
+                                        !pre && R == 1 && in_len <= (M >> 1), tq);
     // the last pass leaves each thread runs of 2^R consecutive positions:
     // store them from registers (no LDS round trip)
     const int64_t count = units >> logR;
@@ -849,6 +982,10 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
     constexpr int VEC = RUN < 4 ? RUN : 4;
 #pragma unroll
     for (int g = 0; g < PSL::G; ++g) {
+        // a truncated transform keeps the positions below 3M/4 (a run lies
+        // within one quarter)
+        if (tq && PSL::index(t, g, 0) >= (3 << (L - 2)))
+            continue;
         W y[RUN];
         fwd_final<W, LOOSE, LOGE, PSL::R, fwd_last_b0<L, LOGE>(2), 0>(x, g * RUN, y, k);
         if (logP >= 2) {
@@ -865,20 +1002,26 @@ __device__ __forceinline__ void fwd_reg_body(const Win *__restrict__ in, int64_t
     }
 }
 
-template <typename Win, typename W, bool LOOSE, int L, int LOGE>
-__global__ void __launch_bounds__(1 << (L - LOGE))
+// TRUNC (3/4-truncated transforms) is a template parameter: its branches,
+// even when not taken, take the radix-32 kernels past 256 registers (one
+// wave per SIMD instead of two).
+template <typename Win, typename W, bool LOOSE, int L, int LOGE, bool TRUNC>
+__global__ void __launch_bounds__(1 << (L - LOGE)) __attribute__((amdgpu_waves_per_eu(LOGE == 5 ? kFwdWaves32 : 1)))
 ntt_fwd_reg(const Win *__restrict__ in, int64_t in_stride, int64_t in_len, W *__restrict__ out,
             int64_t units, int logR, int logP, NttTab<W> tab, W p, int pre)
 {
+    // pre: the column pass's rows as input
     fwd_reg_body<Win, W, LOOSE, L, LOGE>(in, in_stride, in_len, out, units,
-                                         xcd_unit(blockIdx.x, units), logR, logP, tab, p, pre != 0);
+                                         xcd_unit(blockIdx.x, units), logR, logP, tab, p,
+                                         pre != 0, TRUNC);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
 __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__restrict__ out,
                                              int64_t out_stride, int64_t lo, int64_t hi,
                                              W *__restrict__ tmp, int64_t units, int64_t u,
-                                             int logR, int logP, const NttTab<W> &tab, W p)
+                                             int logR, int logP, const NttTab<W> &tab, W p,
+                                             bool trunc = false)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
@@ -890,6 +1033,9 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
     const int64_t N = (int64_t)M << logR;
     const int64_t count = units >> logR;
     constexpr int SLAST = ((L - 1) / LOGE) * LOGE;
+    if (trunc && R == 4 && gg == 3)  // dropped sub-transform (see fwd_reg_body)
+        return;
+    const bool tq = trunc && R == 1;
     W tw[2 * (E - 1)];
     load_pass_tw<W, L, LOGE, SLAST>(tw, t, tab.sitw);
     // the first inverse pass (stages SLAST..L-1) works on runs of 2^R
@@ -900,6 +1046,13 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
     W x[E];
 #pragma unroll
     for (int g = 0; g < PSF::G; ++g) {
+        if (tq && PSF::index(t, g, 0) >= (3 << (L - 2))) {
+            // dropped quarter of a truncated product: never read
+#pragma unroll
+            for (int j = 0; j < RUN; ++j)
+                x[g * RUN + j] = 0;
+            continue;
+        }
         if (logP >= 2) {
 #pragma unroll
             for (int j0 = 0; j0 < RUN; j0 += VEC)
@@ -914,9 +1067,29 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
     }
     const Bfly<W> k(p);
     // inputs below 8p (the pointwise products' lazy outputs)
-    inv_passes<W, LOOSE, L, LOGE, SLAST, 8>(x, tw, lds, t, tab.sitw, k);
+    inv_passes<W, LOOSE, L, LOGE, SLAST, 8>(x, tw, lds, t, tab, k, tq);
     // pass-0 arrangement: element j is index (j << (L-LOGE)) | t
-    if (R == 1) {
+    if (R == 1 && tq) {
+        // truncated: scaled outputs corrected by the dropped quarter
+        // (trunc_correct); register j + q E/4 holds position i + q N/4
+        Wout *o = out + e * out_stride;
+        constexpr int E4 = E >> 2;
+#pragma unroll
+        for (int j = 0; j < E4; ++j) {
+            W c0 = shoup<W>(x[j], tab.ninv, tab.ninvq, p);
+            W c1 = shoup<W>(x[j + E4], tab.ninv, tab.ninvq, p);
+            W c2 = shoup<W>(x[j + 2 * E4], tab.ninv, tab.ninvq, p);
+            trunc_correct<W>(c0, c1, c2, shoup<W>(x[j + 3 * E4], tab.ninv, tab.ninvq, p), tab, p);
+            const int i1 = (j << (L - LOGE)) | t;
+            const int q1 = M >> 2;
+            if (i1 >= lo && i1 < hi)
+                o[i1 - lo] = (Wout)c0;
+            if (i1 + q1 >= lo && i1 + q1 < hi)
+                o[i1 + q1 - lo] = (Wout)c1;
+            if (i1 + 2 * q1 >= lo && i1 + 2 * q1 < hi)
+                o[i1 + 2 * q1 - lo] = (Wout)c2;
+        }
+    } else if (R == 1) {
         Wout *o = out + e * out_stride;
 #pragma unroll
         for (int j = 0; j < E; ++j) {
@@ -939,17 +1112,19 @@ __device__ __forceinline__ void inv_reg_body(const W *__restrict__ in, Wout *__r
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
 __global__ void __launch_bounds__(1 << (L - LOGE)) __attribute__((amdgpu_waves_per_eu(LOGE == 4 ? kInvWaves : 1)))
 ntt_inv_reg(const W *__restrict__ in, Wout *__restrict__ out, int64_t out_stride, int64_t lo,
-            int64_t hi, W *__restrict__ tmp, int64_t units, int logR, int logP, NttTab<W> tab, W p)
+            int64_t hi, W *__restrict__ tmp, int64_t units, int logR, int logP, NttTab<W> tab, W p,
+            int trunc)
 {
     inv_reg_body<W, Wout, LOOSE, L, LOGE>(in, out, out_stride, lo, hi, tmp, units,
-                                          xcd_unit(blockIdx.x, units), logR, logP, tab, p);
+                                          xcd_unit(blockIdx.x, units), logR, logP, tab, p,
+                                          trunc != 0);
 }
 
 // Register-kernel launchers.  logE (elements per thread) is a tuning
 // parameter: 4 (radix-16 passes, more threads, fewer registers) or 5
 // (radix-32 passes, fewer LDS exchanges).
-template <typename Win, typename W, bool LOOSE, int L, int LOGE>
-hipError_t launch_fwd_reg3(const Win *in, int64_t in_stride, int64_t in_len, W *out,
+template <typename Win, typename W, bool LOOSE, int L, int LOGE, bool TRUNC>
+hipError_t launch_fwd_reg4(const Win *in, int64_t in_stride, int64_t in_len, W *out,
                            int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
                            hipStream_t s, int pre)
 {
@@ -957,21 +1132,34 @@ hipError_t launch_fwd_reg3(const Win *in, int64_t in_stride, int64_t in_len, W *
     const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)ntt_fwd_reg<Win, W, LOOSE, L, LOGE>,
+        hipError_t e = hipFuncSetAttribute((const void *)ntt_fwd_reg<Win, W, LOOSE, L, LOGE, TRUNC>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return e;
         attr = true;
     }
-    hipLaunchKernelGGL((ntt_fwd_reg<Win, W, LOOSE, L, LOGE>), dim3((unsigned)units), dim3(T), lds,
-                       s, in, in_stride, in_len, out, units, logR, logP, tab, p, pre);
+    hipLaunchKernelGGL((ntt_fwd_reg<Win, W, LOOSE, L, LOGE, TRUNC>), dim3((unsigned)units), dim3(T),
+                       lds, s, in, in_stride, in_len, out, units, logR, logP, tab, p, pre);
     return hipGetLastError();
+}
+
+// pre: bit 0 the column pass's rows as input, bit 1 a 3/4-truncated transform
+template <typename Win, typename W, bool LOOSE, int L, int LOGE>
+hipError_t launch_fwd_reg3(const Win *in, int64_t in_stride, int64_t in_len, W *out,
+                           int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
+                           hipStream_t s, int pre)
+{
+    if (pre & 2)
+        return launch_fwd_reg4<Win, W, LOOSE, L, LOGE, true>(in, in_stride, in_len, out, units,
+                                                             logR, logP, tab, p, s, pre & 1);
+    return launch_fwd_reg4<Win, W, LOOSE, L, LOGE, false>(in, in_stride, in_len, out, units, logR,
+                                                          logP, tab, p, s, pre & 1);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
 hipError_t launch_inv_reg3(const W *in, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
                            W *tmp, int64_t units, int logR, int logP, const NttTab<W> &tab, W p,
-                           hipStream_t s)
+                           hipStream_t s, int trunc)
 {
     constexpr int T = 1 << (L - LOGE);
     const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
@@ -984,7 +1172,7 @@ hipError_t launch_inv_reg3(const W *in, Wout *out, int64_t out_stride, int64_t l
         attr = true;
     }
     hipLaunchKernelGGL((ntt_inv_reg<W, Wout, LOOSE, L, LOGE>), dim3((unsigned)units), dim3(T), lds,
-                       s, in, out, out_stride, lo, hi, tmp, units, logR, logP, tab, p);
+                       s, in, out, out_stride, lo, hi, tmp, units, logR, logP, tab, p, trunc);
     return hipGetLastError();
 }
 
@@ -1004,14 +1192,14 @@ hipError_t launch_fwd_reg2(int logE, const Win *in, int64_t in_stride, int64_t i
 template <typename W, typename Wout, bool LOOSE, int L>
 hipError_t launch_inv_reg2(int logE, const W *in, Wout *out, int64_t out_stride, int64_t lo,
                            int64_t hi, W *tmp, int64_t units, int logR, int logP,
-                           const NttTab<W> &tab, W p, hipStream_t s)
+                           const NttTab<W> &tab, W p, hipStream_t s, int trunc)
 {
     if (logE == 5 && L >= 12)
         return launch_inv_reg3<W, Wout, LOOSE, L, (L >= 12 ? 5 : 4)>(in, out, out_stride, lo, hi,
                                                                     tmp, units, logR, logP, tab,
-                                                                    p, s);
+                                                                    p, s, trunc);
     return launch_inv_reg3<W, Wout, LOOSE, L, 4>(in, out, out_stride, lo, hi, tmp, units, logR,
-                                                 logP, tab, p, s);
+                                                 logP, tab, p, s, trunc);
 }
 
 template <typename Win, typename W, int L>
@@ -1031,15 +1219,15 @@ hipError_t launch_fwd_reg(int logE, const Win *in, int64_t in_stride, int64_t in
 template <typename W, typename Wout, int L>
 hipError_t launch_inv_reg(int logE, const W *in, Wout *out, int64_t out_stride, int64_t lo,
                           int64_t hi, W *tmp, int64_t units, int logR, int logP,
-                          const NttTab<W> &tab, W p, hipStream_t s)
+                          const NttTab<W> &tab, W p, hipStream_t s, int trunc)
 {
     if constexpr (sizeof(W) == 8) {
         if ((uint64_t)p < ((uint64_t)1 << 60))
             return launch_inv_reg2<W, Wout, true, L>(logE, in, out, out_stride, lo, hi, tmp, units,
-                                                     logR, logP, tab, p, s);
+                                                     logR, logP, tab, p, s, trunc);
     }
     return launch_inv_reg2<W, Wout, false, L>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
-                                              logP, tab, p, s);
+                                              logP, tab, p, s, trunc);
 }
 
 template <typename Win, typename W>
@@ -1070,22 +1258,22 @@ hipError_t dispatch_fwd_reg(int logM, int logE, const Win *in, int64_t in_stride
 template <typename W, typename Wout>
 hipError_t dispatch_inv_reg(int logM, int logE, const W *in, Wout *out, int64_t out_stride,
                             int64_t lo, int64_t hi, W *tmp, int64_t units, int logR, int logP,
-                            const NttTab<W> &tab, W p, hipStream_t s)
+                            const NttTab<W> &tab, W p, hipStream_t s, int trunc = 0)
 {
     switch (logM) {
     case 11:
         return launch_inv_reg<W, Wout, 11>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
-                                           logP, tab, p, s);
+                                           logP, tab, p, s, trunc);
     case 12:
         return launch_inv_reg<W, Wout, 12>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
-                                           logP, tab, p, s);
+                                           logP, tab, p, s, trunc);
     case 13:
         return launch_inv_reg<W, Wout, 13>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
-                                           logP, tab, p, s);
+                                           logP, tab, p, s, trunc);
     case 14:
         if constexpr (sizeof(W) == 4)
             return launch_inv_reg<W, Wout, 14>(logE, in, out, out_stride, lo, hi, tmp, units, logR,
-                                               logP, tab, p, s);
+                                               logP, tab, p, s, trunc);
         [[fallthrough]];
     default:
         return hipErrorInvalidValue;
@@ -1249,19 +1437,41 @@ hipError_t launch_cols_inv(const W *tmp, Wout *out, int64_t out_stride, int64_t 
 
 constexpr int kRegMinLog = 11;
 
+// Stage tables for sub-transforms of 2^logM points.  The tables of length
+// 2^logN are built for min(logN, max_log_m); a truncated transform split in
+// four one level below that (logN = max_log_m + 1) uses the larger table's
+// stages from the second on: stage s of 2^logM points (root w^(N/M)) is
+// stage s + 1 of 2^(logM + 1), which sits 2^logM pairs further on.
+template <typename W>
+NttTab<W> tab_for_sub(const NttTab<W> &tab, int logN, int logM)
+{
+    NttTab<W> t = tab;
+    const int built = logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
+    if (logM < built) {
+        t.stw += (size_t)2 << logM;
+        t.sitw += (size_t)2 << logM;
+    }
+    return t;
+}
+
 }  // namespace
 
 template <typename Win, typename W>
 hipError_t launch_ntt_fwd(const Win *in, int64_t in_stride, int64_t in_len, W *out,
                           int64_t count, int logN, int logP, const NttTune &tune,
                           const NttTab<W> &tab, W p, hipStream_t s, Win *tmp,
-                          const ColTab<W> *ct)
+                          const ColTab<W> *ct, bool trunc)
 {
     if (count <= 0)
         return hipSuccess;
-    const int logM = logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
+    // a truncated transform longer than one sub-transform is split in four
+    const int logM = trunc && logN > NttLimits<W>::max_log_m
+                         ? logN - 2
+                         : (logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m);
     const int logR = logN - logM;
     if (logR > NttLimits<W>::max_log_r || logP > logN)
+        return hipErrorInvalidValue;
+    if (trunc && (logR > 2 || logN < kTruncMinLog || (logR > 0 && logM < kRegMinLog)))
         return hipErrorInvalidValue;
     const int64_t units = count << logR;
     if (logR > 2) {
@@ -1277,42 +1487,49 @@ hipError_t launch_ntt_fwd(const Win *in, int64_t in_stride, int64_t in_len, W *o
     }
     if (logM >= kRegMinLog)
         return dispatch_fwd_reg<Win, W>(logM, tune.fwd_log_e, in, in_stride, in_len, out, units,
-                                        logR, logP, tab, p, s);
+                                        logR, logP, tab_for_sub(tab, logN, logM), p, s,
+                                        trunc ? 2 : 0);
     const int lognb = pick_lognb(logM);
     const int64_t blocks = (units + (1 << lognb) - 1) >> lognb;
     const size_t lds = ((size_t)1 << (logM + lognb)) * sizeof(W);
     hipLaunchKernelGGL((ntt_fwd_kernel<Win, W>), dim3((unsigned)blocks), dim3(kThreads), lds, s, in,
-                       in_stride, in_len, out, units, logN, logR, lognb, logP, tab, p);
+                       in_stride, in_len, out, units, logN, logR, lognb, logP, tab, p, trunc ? 1 : 0);
     return hipGetLastError();
 }
 
 template <typename W, typename Wout>
 hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo, int64_t hi,
                           int64_t count, int logN, int logP, const NttTune &tune,
-                          const NttTab<W> &tab, W p, W *tmp, hipStream_t s, const ColTab<W> *ct)
+                          const NttTab<W> &tab, W p, W *tmp, hipStream_t s, const ColTab<W> *ct,
+                          bool trunc)
 {
     const int64_t N = (int64_t)1 << logN;
     if (hi > N)
         hi = N;
     if (count <= 0 || hi <= lo)
         return hipSuccess;
-    const int logM = logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
+    const int logM = trunc && logN > NttLimits<W>::max_log_m
+                         ? logN - 2
+                         : (logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m);
     const int logR = logN - logM;
     if (logR > NttLimits<W>::max_log_r || (logR > 0 && tmp == nullptr) || logP > logN ||
-        (logR > 2 && (!ct || logM < kRegMinLog)))
+        (logR > 2 && (!ct || logM < kRegMinLog)) ||
+        (trunc && (logR > 2 || logN < kTruncMinLog || hi > 3 * (N >> 2) ||
+                   (logR > 0 && logM < kRegMinLog))))
         return hipErrorInvalidValue;
     const int64_t units = count << logR;
     hipError_t err;
     if (logM >= kRegMinLog) {
         err = dispatch_inv_reg<W, Wout>(logM, tune.inv_log_e, in, out, out_stride, lo, hi, tmp,
-                                        units, logR, logP, tab, p, s);
+                                        units, logR, logP, tab_for_sub(tab, logN, logM), p, s,
+                                        trunc ? 1 : 0);
     } else {
         const int lognb = pick_lognb(logM);
         const int64_t blocks = (units + (1 << lognb) - 1) >> lognb;
         const size_t lds = ((size_t)1 << (logM + lognb)) * sizeof(W);
         hipLaunchKernelGGL((ntt_inv_kernel<W, Wout>), dim3((unsigned)blocks), dim3(kThreads), lds,
                            s, in, out, out_stride, lo, hi, tmp, units, logN, logR, lognb, logP,
-                           tab, p);
+                           tab, p, trunc ? 1 : 0);
         err = hipGetLastError();
     }
     if (err != hipSuccess || logR == 0)
@@ -1320,6 +1537,14 @@ hipError_t launch_ntt_inv(const W *in, Wout *out, int64_t out_stride, int64_t lo
     if (logR > 2)
         return launch_cols_inv<W, Wout>(tmp, out, out_stride, lo, hi, count, logM, logR, *ct,
                                         tab.ninv, tab.ninvq, p, s);
+    if (trunc) {
+        int64_t tblocks = ((count << (logN - 2)) + kThreads - 1) / kThreads;
+        if (tblocks > 65536)
+            tblocks = 65536;
+        hipLaunchKernelGGL((ntt_merge_trunc_kernel<W, Wout>), dim3((unsigned)tblocks), dim3(kThreads),
+                           0, s, tmp, out, out_stride, lo, hi, count, logN, tab, p);
+        return hipGetLastError();
+    }
     const int64_t total = count * (hi - lo);
     int64_t mblocks = (total + kThreads - 1) / kThreads;
     if (mblocks > 65536)
@@ -1374,7 +1599,7 @@ ntt_fwd_segs_lds(const FwdSeg<Win, W> *__restrict__ segs, int nseg, int logN, in
     const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
     const FwdSeg<Win, W> &sg = seg_of(segs, nseg, b);
     fwd_lds_body<Win, W>(sg.in, sg.in_stride, sg.in_len, sg.out, sg.count << logR,
-                         (b - sg.block0) << lognb, logN, logR, lognb, logP, tab, p);
+                         (b - sg.block0) << lognb, logN, logR, lognb, logP, tab, p, sg.trunc != 0);
 }
 
 template <typename W, typename Wout>
@@ -1385,18 +1610,19 @@ ntt_inv_segs_lds(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logN, i
     const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
     const InvSeg<W, Wout> &sg = seg_of(segs, nseg, b);
     inv_lds_body<W, Wout>(sg.in, sg.out, sg.out_stride, sg.lo, sg.hi, sg.tmp, sg.count << logR,
-                          (b - sg.block0) << lognb, logN, logR, lognb, logP, tab, p);
+                          (b - sg.block0) << lognb, logN, logR, lognb, logP, tab, p, sg.trunc != 0);
 }
 
-template <typename Win, typename W, bool LOOSE, int L, int LOGE>
-__global__ void __launch_bounds__(1 << (L - LOGE))
+// every segment of one launch has the launch's TRUNC (see ntt_fwd_reg)
+template <typename Win, typename W, bool LOOSE, int L, int LOGE, bool TRUNC>
+__global__ void __launch_bounds__(1 << (L - LOGE)) __attribute__((amdgpu_waves_per_eu(LOGE == 5 ? kFwdWaves32 : 1)))
 ntt_fwd_segs_reg(const FwdSeg<Win, W> *__restrict__ segs, int nseg, int logR, int logP,
                  NttTab<W> tab, W p)
 {
     const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
     const FwdSeg<Win, W> &sg = seg_of(segs, nseg, b);
     fwd_reg_body<Win, W, LOOSE, L, LOGE>(sg.in, sg.in_stride, sg.in_len, sg.out, sg.count << logR,
-                                         b - sg.block0, logR, logP, tab, p);
+                                         b - sg.block0, logR, logP, tab, p, false, TRUNC);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
@@ -1407,7 +1633,8 @@ ntt_inv_segs_reg(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logR, i
     const int64_t b = xcd_unit(blockIdx.x, gridDim.x);
     const InvSeg<W, Wout> &sg = seg_of(segs, nseg, b);
     inv_reg_body<W, Wout, LOOSE, L, LOGE>(sg.in, sg.out, sg.out_stride, sg.lo, sg.hi, sg.tmp,
-                                          sg.count << logR, b - sg.block0, logR, logP, tab, p);
+                                          sg.count << logR, b - sg.block0, logR, logP, tab, p,
+                                          sg.trunc != 0);
 }
 
 template <typename W, typename Wout>
@@ -1416,8 +1643,14 @@ ntt_merge_segs(const InvSeg<W, Wout> *__restrict__ segs, int nseg, int logN, int
                NttTab<W> tab, W p)
 {
     const InvSeg<W, Wout> &sg = mseg_of(segs, nseg, blockIdx.x);
-    const int64_t width = sg.hi - sg.lo;
     const int64_t idx = ((int64_t)blockIdx.x - sg.mblock0) * kThreads + threadIdx.x;
+    if (sg.trunc) {
+        if (idx < (sg.count << (logN - 2)))
+            merge_trunc_one<W, Wout>(sg.tmp, sg.out, sg.out_stride, sg.lo, sg.hi, idx, logN, tab,
+                                     p);
+        return;
+    }
+    const int64_t width = sg.hi - sg.lo;
     if (idx < sg.count * width)
         merge_one<W, Wout>(sg.tmp, sg.out, sg.out_stride, sg.lo, width, idx, logN, logR, tab, p);
 }
@@ -1434,18 +1667,28 @@ hipError_t set_lds_attr(Kern *k, size_t lds, bool &done)
     return e;
 }
 
-template <typename Win, typename W, bool LOOSE, int L, int LOGE>
-hipError_t fwd_segs_reg3(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR, int logP,
+template <typename Win, typename W, bool LOOSE, int L, int LOGE, bool TRUNC>
+hipError_t fwd_segs_reg4(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR, int logP,
                          const NttTab<W> &tab, W p, hipStream_t s)
 {
     const size_t lds = (size_t)((1 << L) + (1 << L) / 32) * sizeof(W);
     static bool attr = false;
-    hipError_t e = set_lds_attr(ntt_fwd_segs_reg<Win, W, LOOSE, L, LOGE>, lds, attr);
+    hipError_t e = set_lds_attr(ntt_fwd_segs_reg<Win, W, LOOSE, L, LOGE, TRUNC>, lds, attr);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL((ntt_fwd_segs_reg<Win, W, LOOSE, L, LOGE>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((ntt_fwd_segs_reg<Win, W, LOOSE, L, LOGE, TRUNC>), dim3((unsigned)blocks),
                        dim3(1 << (L - LOGE)), lds, s, segs, nseg, logR, logP, tab, p);
     return hipGetLastError();
+}
+
+template <typename Win, typename W, bool LOOSE, int L, int LOGE>
+hipError_t fwd_segs_reg3(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR, int logP,
+                         const NttTab<W> &tab, W p, hipStream_t s, bool trunc)
+{
+    if (trunc)
+        return fwd_segs_reg4<Win, W, LOOSE, L, LOGE, true>(segs, nseg, blocks, logR, logP, tab, p,
+                                                           s);
+    return fwd_segs_reg4<Win, W, LOOSE, L, LOGE, false>(segs, nseg, blocks, logR, logP, tab, p, s);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L, int LOGE>
@@ -1464,12 +1707,12 @@ hipError_t inv_segs_reg3(const InvSeg<W, Wout> *segs, int nseg, int64_t blocks, 
 
 template <typename Win, typename W, bool LOOSE, int L>
 hipError_t fwd_segs_reg2(int logE, const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR,
-                         int logP, const NttTab<W> &tab, W p, hipStream_t s)
+                         int logP, const NttTab<W> &tab, W p, hipStream_t s, bool trunc)
 {
     if (logE == 5 && L >= 12)
         return fwd_segs_reg3<Win, W, LOOSE, L, (L >= 12 ? 5 : 4)>(segs, nseg, blocks, logR, logP,
-                                                                 tab, p, s);
-    return fwd_segs_reg3<Win, W, LOOSE, L, 4>(segs, nseg, blocks, logR, logP, tab, p, s);
+                                                                 tab, p, s, trunc);
+    return fwd_segs_reg3<Win, W, LOOSE, L, 4>(segs, nseg, blocks, logR, logP, tab, p, s, trunc);
 }
 
 template <typename W, typename Wout, bool LOOSE, int L>
@@ -1484,13 +1727,14 @@ hipError_t inv_segs_reg2(int logE, const InvSeg<W, Wout> *segs, int nseg, int64_
 
 template <typename Win, typename W, int L>
 hipError_t fwd_segs_reg(int logE, const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logR,
-                        int logP, const NttTab<W> &tab, W p, hipStream_t s)
+                        int logP, const NttTab<W> &tab, W p, hipStream_t s, bool trunc)
 {
     if constexpr (sizeof(W) == 8) {
         if ((uint64_t)p < ((uint64_t)1 << 60))
-            return fwd_segs_reg2<Win, W, true, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
+            return fwd_segs_reg2<Win, W, true, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s,
+                                                  trunc);
     }
-    return fwd_segs_reg2<Win, W, false, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s);
+    return fwd_segs_reg2<Win, W, false, L>(logE, segs, nseg, blocks, logR, logP, tab, p, s, trunc);
 }
 
 template <typename W, typename Wout, int L>
@@ -1505,17 +1749,20 @@ hipError_t inv_segs_reg(int logE, const InvSeg<W, Wout> *segs, int nseg, int64_t
 }
 
 template <typename W>
-int seg_log_m(int logN)
+int seg_log_m(int logN, bool trunc)
 {
+    // a truncated product longer than one sub-transform is split in four
+    if (trunc && logN > NttLimits<W>::max_log_m)
+        return logN - 2;
     return logN < NttLimits<W>::max_log_m ? logN : NttLimits<W>::max_log_m;
 }
 
 }  // namespace
 
 template <typename W>
-int64_t ntt_seg_blocks(int64_t count, int logN)
+int64_t ntt_seg_blocks(int64_t count, int logN, bool trunc)
 {
-    const int logM = seg_log_m<W>(logN);
+    const int logM = seg_log_m<W>(logN, trunc);
     const int64_t units = count << (logN - logM);
     if (logM >= kRegMinLog)
         return units;
@@ -1524,34 +1771,42 @@ int64_t ntt_seg_blocks(int64_t count, int logN)
 }
 
 template <typename W>
-int64_t merge_seg_blocks(int64_t count, int64_t width, int logN)
+int64_t merge_seg_blocks(int64_t count, int64_t width, int logN, bool trunc)
 {
     if (logN <= NttLimits<W>::max_log_m || width <= 0)
         return 0;
+    if (trunc)  // one item per quadruple of outputs
+        return ((count << (logN - 2)) + kThreads - 1) / kThreads;
     return (count * width + kThreads - 1) / kThreads;
 }
 
 template <typename Win, typename W>
 hipError_t launch_ntt_fwd_segs(const FwdSeg<Win, W> *segs, int nseg, int64_t blocks, int logN,
-                               int logP, int log_e, const NttTab<W> &tab, W p, hipStream_t s)
+                               int logP, int log_e, const NttTab<W> &tab, W p, hipStream_t s,
+                               bool trunc)
 {
     if (nseg <= 0 || blocks <= 0)
         return hipSuccess;
-    const int logM = seg_log_m<W>(logN);
+    const int logM = seg_log_m<W>(logN, trunc);
     const int logR = logN - logM;
     if (logR > 2 || logP > logN || blocks > INT32_MAX)
         return hipErrorInvalidValue;
+    const NttTab<W> st = tab_for_sub(tab, logN, logM);
     if (logM >= kRegMinLog) {
         switch (logM) {
         case 11:
-            return fwd_segs_reg<Win, W, 11>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            return fwd_segs_reg<Win, W, 11>(log_e, segs, nseg, blocks, logR, logP, st, p, s,
+                                             trunc);
         case 12:
-            return fwd_segs_reg<Win, W, 12>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            return fwd_segs_reg<Win, W, 12>(log_e, segs, nseg, blocks, logR, logP, st, p, s,
+                                             trunc);
         case 13:
-            return fwd_segs_reg<Win, W, 13>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            return fwd_segs_reg<Win, W, 13>(log_e, segs, nseg, blocks, logR, logP, st, p, s,
+                                             trunc);
         case 14:
             if constexpr (sizeof(W) == 4)
-                return fwd_segs_reg<Win, W, 14>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+                return fwd_segs_reg<Win, W, 14>(log_e, segs, nseg, blocks, logR, logP, st, p, s,
+                                                 trunc);
             [[fallthrough]];
         default:
             return hipErrorInvalidValue;
@@ -1567,29 +1822,30 @@ hipError_t launch_ntt_fwd_segs(const FwdSeg<Win, W> *segs, int nseg, int64_t blo
 template <typename W, typename Wout>
 hipError_t launch_ntt_inv_segs(const InvSeg<W, Wout> *segs, int nseg, int64_t blocks,
                                int64_t mblocks, int logN, int logP, int log_e,
-                               const NttTab<W> &tab, W p, hipStream_t s)
+                               const NttTab<W> &tab, W p, hipStream_t s, bool trunc)
 {
     if (nseg <= 0 || blocks <= 0)
         return hipSuccess;
-    const int logM = seg_log_m<W>(logN);
+    const int logM = seg_log_m<W>(logN, trunc);
     const int logR = logN - logM;
     if (logR > 2 || logP > logN || blocks > INT32_MAX || mblocks > INT32_MAX)
         return hipErrorInvalidValue;
+    const NttTab<W> st = tab_for_sub(tab, logN, logM);
     hipError_t err;
     if (logM >= kRegMinLog) {
         switch (logM) {
         case 11:
-            err = inv_segs_reg<W, Wout, 11>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            err = inv_segs_reg<W, Wout, 11>(log_e, segs, nseg, blocks, logR, logP, st, p, s);
             break;
         case 12:
-            err = inv_segs_reg<W, Wout, 12>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            err = inv_segs_reg<W, Wout, 12>(log_e, segs, nseg, blocks, logR, logP, st, p, s);
             break;
         case 13:
-            err = inv_segs_reg<W, Wout, 13>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+            err = inv_segs_reg<W, Wout, 13>(log_e, segs, nseg, blocks, logR, logP, st, p, s);
             break;
         case 14:
             if constexpr (sizeof(W) == 4) {
-                err = inv_segs_reg<W, Wout, 14>(log_e, segs, nseg, blocks, logR, logP, tab, p, s);
+                err = inv_segs_reg<W, Wout, 14>(log_e, segs, nseg, blocks, logR, logP, st, p, s);
                 break;
             }
             [[fallthrough]];
@@ -1610,46 +1866,46 @@ hipError_t launch_ntt_inv_segs(const InvSeg<W, Wout> *segs, int nseg, int64_t bl
     return hipGetLastError();
 }
 
-template int64_t ntt_seg_blocks<uint64_t>(int64_t, int);
-template int64_t ntt_seg_blocks<uint32_t>(int64_t, int);
-template int64_t merge_seg_blocks<uint64_t>(int64_t, int64_t, int);
-template int64_t merge_seg_blocks<uint32_t>(int64_t, int64_t, int);
+template int64_t ntt_seg_blocks<uint64_t>(int64_t, int, bool);
+template int64_t ntt_seg_blocks<uint32_t>(int64_t, int, bool);
+template int64_t merge_seg_blocks<uint64_t>(int64_t, int64_t, int, bool);
+template int64_t merge_seg_blocks<uint32_t>(int64_t, int64_t, int, bool);
 template hipError_t launch_ntt_fwd_segs<uint64_t, uint64_t>(const FwdSeg<uint64_t, uint64_t> *, int,
                                                             int64_t, int, int, int,
                                                             const NttTab<uint64_t> &, uint64_t,
-                                                            hipStream_t);
+                                                            hipStream_t, bool);
 template hipError_t launch_ntt_fwd_segs<uint64_t, uint32_t>(const FwdSeg<uint64_t, uint32_t> *, int,
                                                             int64_t, int, int, int,
                                                             const NttTab<uint32_t> &, uint32_t,
-                                                            hipStream_t);
+                                                            hipStream_t, bool);
 template hipError_t launch_ntt_inv_segs<uint64_t, uint64_t>(const InvSeg<uint64_t, uint64_t> *, int,
                                                             int64_t, int64_t, int, int, int,
                                                             const NttTab<uint64_t> &, uint64_t,
-                                                            hipStream_t);
+                                                            hipStream_t, bool);
 template hipError_t launch_ntt_inv_segs<uint32_t, uint64_t>(const InvSeg<uint32_t, uint64_t> *, int,
                                                             int64_t, int64_t, int, int, int,
                                                             const NttTab<uint32_t> &, uint32_t,
-                                                            hipStream_t);
+                                                            hipStream_t, bool);
 
 template hipError_t launch_ntt_fwd<uint64_t, uint64_t>(const uint64_t *, int64_t, int64_t,
                                                        uint64_t *, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint64_t> &,
                                                        uint64_t, hipStream_t, uint64_t *,
-                                                       const ColTab<uint64_t> *);
+                                                       const ColTab<uint64_t> *, bool);
 template hipError_t launch_ntt_fwd<uint64_t, uint32_t>(const uint64_t *, int64_t, int64_t,
                                                        uint32_t *, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint32_t> &,
                                                        uint32_t, hipStream_t, uint64_t *,
-                                                       const ColTab<uint32_t> *);
+                                                       const ColTab<uint32_t> *, bool);
 template hipError_t launch_ntt_inv<uint64_t, uint64_t>(const uint64_t *, uint64_t *, int64_t,
                                                        int64_t, int64_t, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint64_t> &,
                                                        uint64_t, uint64_t *, hipStream_t,
-                                                       const ColTab<uint64_t> *);
+                                                       const ColTab<uint64_t> *, bool);
 template hipError_t launch_ntt_inv<uint32_t, uint64_t>(const uint32_t *, uint64_t *, int64_t,
                                                        int64_t, int64_t, int64_t, int, int,
                                                        const NttTune &, const NttTab<uint32_t> &,
                                                        uint32_t, uint32_t *, hipStream_t,
-                                                       const ColTab<uint32_t> *);
+                                                       const ColTab<uint32_t> *, bool);
 
 }  // namespace pmlg

@@ -63,6 +63,30 @@ def test_mul_matches_oracle(pml, p, method, case):
     assert np.array_equal(C, O.polmat_mul(A, B, p))
 
 
+@pytest.mark.parametrize("p,method", [(P60, "plain"), (P60, "mfma"), (P30, "plain"), (P30, "mfma"),
+                                      (P62, "plain"), (7340033, "mfma")])
+def test_mul_truncated_transforms(pml, p, method):
+    """Products of length at most 3N/4 evaluate at the first 3N/4 positions
+    of the transform only (the DIF outputs at the frequencies 0, 1, 2 mod 4)
+    and rebuild the top quarter from the known zero coefficients: both
+    boundaries (N/2 + 1 and 3N/4) and one length just above (a full
+    transform), for the LDS kernels (N < 2^11), the register kernels and the
+    split transforms."""
+    rng = np.random.default_rng(p % 1000 + len(method))
+    # up to 4 times the longest sub-transform (2^13 points of 64 bits, 2^14
+    # of 32 bits): beyond one sub-transform the transform is split in four and
+    # the fourth sub-transform dropped
+    logs = [5, 6, 9, 10, 11, 12, 13, 14, 15] + ([16] if p < (1 << 30) else [])
+    for lg in logs:
+        N = 1 << lg
+        for lenc in (N // 2 + 1, 3 * N // 4 - 1, 3 * N // 4, 3 * N // 4 + 1):
+            la = lenc // 3 + 1
+            lb = lenc - la + 1
+            A, B = rand(rng, (2, 3, la), p), rand(rng, (3, 2, lb), p)
+            C = ctx_for(pml, p).mul(A, B, method=method)
+            assert np.array_equal(C, O.polmat_mul(A, B, p, method="ntt")), (lg, lenc)
+
+
 @pytest.mark.parametrize("method", ["plain", "mfma"])
 def test_mul_large_length_split(pml, method):
     """Transform lengths above the LDS limit (radix-2/4 split + merge)."""
