@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     int *pivcol = pos + m;                                           // kMaxN (by rank)
     int *prow = pivcol + kMaxN;                                      // kMaxN (by rank)
     int *ispiv = prow + kMaxN;                                       // kMaxN (rows of At)
-    __shared__ int s_nb[2], s_mask[2], s_prs[2][kPanel];
+    __shared__ int s_nb[2], s_prs[2][kPanel];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint64_t p = a.mc.p;
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 for (int s2 = 1; s2 <= P; ++s2)
                     cc[h][s2] = 0;
             }
-            int nb = 0, mask = 0;
+            int nb = 0;
             // one pivot step on panel column j (a compile-time index: the
             // convergent ballots and lane reads keep the compiler from unrolling
             // a loop around them, and the arrays must stay in registers)
@@ -567,7 +567,6 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                         cc[h][s2] = ar.msub(pv, cc[h][s2], fr, qc[s2]);
                 }
                 prs[j] = pr;
-                mask |= 1 << j;
                 if (lane == 0) {
                     prow[rank + nb] = pr;
                     pivcol[rank + nb] = c0 + j;
@@ -596,7 +595,6 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             }
             if (lane == 0) {
                 s_nb[buf] = nb;
-                s_mask[buf] = mask;
 #pragma unroll
                 for (int k = 0; k < P; ++k)
                     s_prs[buf][k] = prs[k];
@@ -605,26 +603,36 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         __builtin_amdgcn_s_setprio(0);
     };
     // apply the combination in buffer buf to this wave's columns: the pivot
-    // rows' values are read before any row is written
-    auto apply = [&](const int buf, const int mask) {
-        const int G = 64 / cw;
-        const int cl = lane % cw, g = lane / cw;
-        const int cg = wave * cw + cl;
-        if (cg >= m)
+    // rows' values are read before any row is written.  Every term is
+    // formed: a panel column without a pivot has row 0 in s_prs and zero
+    // coefficients in every row, so its term is zero, and issuing all loads
+    // before the products (no branch between them) keeps the LDS round
+    // trips of a panel to three instead of one per term.
+    const int acl = lane & (cw - 1), ag = lane / cw, aG = 64 / cw;  // cw: a power of two
+    const int acg = wave * cw + acl;
+    auto apply = [&](const int buf) {
+        if (acg >= m)
             return;
+        int prq[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            prq[q] = s_prs[buf][q];
         V pvr[P];
 #pragma unroll
         for (int q = 0; q < P; ++q)
-            pvr[q] = mask & (1 << q) ? At[s_prs[buf][q] * ms + cg] : (V)0;
+            pvr[q] = At[prq[q] * ms + acg];
         const V *C = Ck + buf * kMaxN * (P + 1);
-        for (int r = g; r < n; r += G) {
+        for (int r = ag; r < n; r += aG) {
             const V *cr = C + r * (P + 1);
-            V *x = At + r * ms + cg;
-            typename Arith::Acc acc = ar.prod(cr[0], *x);
+            V *x = At + r * ms + acg;
+            V cv[P + 1];
+#pragma unroll
+            for (int s2 = 0; s2 <= P; ++s2)
+                cv[s2] = cr[s2];
+            typename Arith::Acc acc = ar.prod(cv[0], *x);
 #pragma unroll
             for (int q = 0; q < P; ++q)
-                if (mask & (1 << q))
-                    acc = ar.fma(acc, cr[q + 1], pvr[q]);
+                acc = ar.fma(acc, cv[q + 1], pvr[q]);
             *x = ar.reduce(acc);
         }
     };
@@ -633,9 +641,9 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     __syncthreads();
     for (int k = 0;; ++k) {
         const int buf = k & 1;
-        const int nb = s_nb[buf], mask = s_mask[buf];
+        const int nb = s_nb[buf];
         if (nb > 0)
-            apply(buf, mask);
+            apply(buf);
         rank += nb;
         const int c1 = (k + 1) * P;
         if (c1 >= m || rank >= n)
