@@ -320,7 +320,8 @@ def main():
         key = "%s_batch%d_%s" % (names[dom], Bn, variant[names[dom]])
         if key in tr:
             traffic = tr[key]["hbm_bytes_per_launch"]
-    kernels = {"forward": "ntt_fwd_segs_reg<u64,u64,loose,L=13,LOGE=%d>" % fe,
+    # (the C2 product is not truncated: lenC = 8191 > 3N/4)
+    kernels = {"forward": "ntt_fwd_segs_reg<u64,u64,loose,L=13,LOGE=%d,TRUNC=0>" % fe,
                "pointwise": ("pw_mfma_segs<u64,8,%d>" % (4 >> pbs)) if kind == "mfma"
                else "pw_plain_segs<u64>",
                "inverse": "ntt_inv_segs_reg<u64,u64,loose,L=13,LOGE=%d>" % ie}

@@ -329,7 +329,7 @@ void pmlg_mlp_features(int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t le
 pmlg_status pmlg_dispatch_set_model(pmlg_mlp mlp);
 /* Measured refinement of the model's choice (on by default; PMLG_REFINE=0 in
  * the environment or this call turns it off): when configurations other
- * than the model's best are predicted within 5% of it, those candidates are
+ * than the model's best are predicted within 15% of it, those candidates are
  * timed once per shape class, on replicated products as in pmlg_autotune,
  * and the fastest is used for every later product of the class. */
 void pmlg_dispatch_set_refine(int on);

@@ -34,8 +34,14 @@ int choose_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, i
 int learned_config(pmlg_ctx ctx, int64_t m, int64_t k, int64_t n, int64_t lenA, int64_t lenB,
                    float *pred = nullptr);
 // The model's top candidates whose predicted times lie within kRefineMargin
-// of its best are timed once per shape class (measured refinement).
-constexpr double kRefineMargin = 0.05;
+// of its best are timed once per shape class (measured refinement).  The
+// network's error is about 24% RMS in log time and not uniform across
+// configurations (smooth features cannot show the MFMA tiles' padding steps
+// at multiples of 32), so the margin is wide: on 2,000 held-out shapes, with
+// the recorded times standing in for the live ones, 15% brings the total
+// within 0.05% of the per-shape best with 4 candidates a shape (5%: 0.3%
+// above it with 2.6).
+constexpr double kRefineMargin = 0.15;
 bool refine_enabled();
 
 pmlg_status build_plan(pmlg_batch_plan *out, const pmlg_mul_desc *descs, int64_t count, int flags,

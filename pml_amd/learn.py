@@ -286,6 +286,8 @@ def main(argv=None):
     c.add_argument("--shapes", type=int, default=22000)
     c.add_argument("--seed", type=int, default=2024)
     c.add_argument("--reps", type=int, default=1)
+    c.add_argument("--range", default=None,
+                   help="start:stop, a slice of the drawn shapes (collections split over runs)")
     c.add_argument("--out", required=True)
     bd = sub.add_parser("bands")
     bd.add_argument("--data", default=None)
@@ -301,6 +303,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "collect":
         shapes = draw_shapes(a.shapes, a.seed)
+        if a.range:
+            lo, hi = (int(v) for v in a.range.split(":"))
+            shapes = shapes[lo:hi]
         t0 = time.time()
         T = collect(shapes, reps=a.reps, progress=500)
         np.savez(a.out, shapes=np.array(shapes, dtype=np.uint64), times=T)

@@ -113,10 +113,14 @@ def test_learned_dispatch_picks_argmin_and_stays_exact(learn):
             c.close()
 
 
+REFINE_MARGIN = 0.15  # kRefineMargin, pml_amd/csrc/dispatch.hpp
+
+
 def test_refinement_picks_among_close_candidates(learn):
-    """With refinement on, a shape whose predicted times lie within 5% of
-    the model's best gets one of those candidates, the same one on every later
-    call (cached per shape class), and exact products."""
+    """With refinement on, a shape whose predicted times lie within the
+    refinement margin (kRefineMargin in dispatch.hpp, 15%) of the model's best
+    gets one of those candidates, the same one on every later call (cached
+    per shape class), and exact products."""
     import math
     import pml_amd
     mlp = learn.Mlp.load(learn.DEFAULT_MODEL)
@@ -130,7 +134,7 @@ def test_refinement_picks_among_close_candidates(learn):
             pred = mlp.forward(learn.features(m, k, n, la, lb, P60)[None, :])[0]
             valid = [c for c in range(8) if pml_amd.config_valid(ctx, m, k, n, la, lb, c)]
             best = min(pred[c] for c in valid)
-            cand = {c for c in valid if pred[c] <= best + math.log1p(0.05) + 1e-6}
+            cand = {c for c in valid if pred[c] <= best + math.log1p(REFINE_MARGIN) + 1e-6}
             got = pml_amd.choose_config(ctx, m, k, n, la, lb, "auto")
             assert got in cand, (got, cand)
             assert pml_amd.choose_config(ctx, m, k, n, la, lb, "auto") == got
