@@ -832,8 +832,20 @@ __device__ __forceinline__ uint64_t row_combo(const uint64_t *src, const LeafArg
 template <bool SMALL, bool BIG>
 __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
 {
+    // The control words, the step's row list and row roles and its factor D
+    // are all requested at entry, before any branch on them: one round trip
+    // to memory instead of three dependent ones (mode, then the row lists,
+    // then rank, length and D).  In modes 0 and 2 the lists and D are stale
+    // and unused.
+    static_assert(kMaxM == 256, "one row-list entry per thread of the 256-thread workgroup");
     const int mode = a.ctl[C_MODE];
     const int cur = a.ctl[C_CUR];
+    const int rank = a.ctl[C_RANK];
+    const int L = a.ctl[C_LEN];
+    const uint64_t D = a.scale[a.m];
+    const int tx = threadIdx.x;
+    const int piv_t = tx < a.m ? a.piv[tx] : 0;  // groups read up to kMaxM entries
+    const int rinfo_t = tx < a.m ? a.rowinfo[tx] : 0;
     const unsigned tstride = gridDim.x * blockDim.x;
     const unsigned t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (mode == 0) {
@@ -861,15 +873,10 @@ __global__ void __launch_bounds__(256) mbasis_update_kernel(LeafArgs a, int ord)
         return;
     }
     __shared__ int piv[kMaxM], rinfo[kMaxM];
-    for (int i = threadIdx.x; i < kMaxM; i += blockDim.x) {
-        piv[i] = i < m ? a.piv[i] : 0;  // groups read up to kMaxM entries
-        rinfo[i] = i < m ? a.rowinfo[i] : 0;
-    }
+    piv[tx] = piv_t;
+    rinfo[tx] = rinfo_t;
     __syncthreads();
-    const int rank = a.ctl[C_RANK];
-    const int L = a.ctl[C_LEN];
-    // the kernel vectors' own coefficient D (the step's pivot product)
-    const uint64_t D = a.scale[m];
+    // D: the kernel vectors' own coefficient (the step's pivot product)
     auto own = [&](uint64_t x) -> uint64_t {
         if constexpr (SMALL)  // D in Montgomery form: REDC(x D 2^32) = x D mod p
             return Mont32Arith(a.mc).redc((uint64_t)(uint32_t)x * (uint32_t)D);
