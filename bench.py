@@ -154,6 +154,7 @@ def bench_c4(reps):
     from pml_amd import learn, lib, _check
     from pml_amd.perf import DeviceBatch
     shapes = learn.draw_shapes(512, 4)
+    model = learn.install_default()  # the learned leg's model (C2's is closed by now)
     batch = DeviceBatch(shapes, "auto")
     out = {"workload": "512 uneven products, m,k,n in 2..64, lengths 16..16384 log-uniform, "
                        "p in {998244353, 2^60-3*2^24+1}, seed 4",
@@ -178,6 +179,7 @@ def bench_c4(reps):
         out[name] = {"seconds_per_batch": round(dt, 5), "products_per_s": round(512 / dt, 1),
                      "plan_seconds": round(plan_s, 2), "launches": launches.value}
     batch.close()
+    model.close()
     return out
 
 
@@ -254,7 +256,7 @@ def main():
     ctx = pml_amd.Context(P60)
     model = learn.install_default()
     # the network's own pick, and the pick after the measured refinement among
-    # the configurations it predicts within 5% of its best
+    # the configurations it predicts within the refinement margin of its best
     pml_amd.set_refine(False)
     model_choice = pml_amd.choose_config(ctx, M, K, N_, LEN, LEN, "auto")
     pml_amd.set_refine(True)
