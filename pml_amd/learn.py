@@ -14,7 +14,8 @@ in a real plan).
 forward, backward and Adam kernels) on log(ms) of the first shapes and
 evaluates on the held-out rest: total recorded time of the configurations the
 network picks vs those of the threshold rule, and vs the per-shape best.
-`bands` fits the banded threshold rule (the fallback and the judge) on the
+`bands` fits the banded threshold rule (the fallback, and the baseline the
+network is compared with) on the
 same training shapes and writes its table for the library.
 """
 import argparse
@@ -192,7 +193,7 @@ def dataset(shapes, T):
 
 
 # ---------------------------------------------------------------------------
-# banded threshold rule (the fallback and the judge of the learned choice)
+# banded threshold rule (the fallback, and the baseline of the learned choice)
 # ---------------------------------------------------------------------------
 # Bands in the style of the reference's dispatch (mul.c:57-96, bands over the
 # dimension and the length): word size (p < 2^30 or not), the shorter
