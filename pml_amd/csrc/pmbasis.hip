@@ -499,6 +499,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             V pn[H][P], cc[H][P + 1];
             bool isp[H], ok[H];
             int prs[P] = {0};
+            int tk = 0;  // panel columns that took a pivot
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 const int r = lane + 64 * h;
@@ -527,51 +528,48 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                     return;  // no pivot in this column
                 const int pr = b0 ? __ffsll(b0) - 1 : 63 + __ffsll(b1);
                 const int pl = pr & 63, ph = pr >> 6;
-                // the pivot row's self coefficient moves to slot j + 1 (ph is
-                // uniform: explicit branches keep the arrays in registers)
-#pragma unroll
-                for (int h = 0; h < H; ++h) {
-                    if (lane == pl && ph == h) {
-                        cc[h][j + 1] = cc[h][0];
-                        cc[h][0] = 0;
-                        isp[h] = true;
-                    }
-                }
                 // broadcast the pivot row (panel values and coefficients): the
                 // pivot lane is uniform, so its registers are read into
-                // scalar registers
+                // scalar registers; its self coefficient cc[0] moves to slot
+                // j + 1
                 V qp[P], qc[P + 1];
 #pragma unroll
                 for (int jj = j; jj < P; ++jj)
                     qp[jj] = lane_value(H == 2 && ph ? pn[H - 1][jj] : pn[0][jj], pl);
 #pragma unroll
-                for (int s2 = 1; s2 <= j + 1; ++s2)
+                for (int s2 = 1; s2 <= j; ++s2)
                     qc[s2] = lane_value(H == 2 && ph ? cc[H - 1][s2] : cc[0][s2], pl);
+                qc[j + 1] = lane_value(H == 2 && ph ? cc[H - 1][0] : cc[0][0], pl);
                 const V pv = qp[j];
                 // Only the nonzero terms are formed: column j of every other
-                // row becomes pv * fr - fr * pv = 0, the pivot row's slot 0 is
-                // zero after the move, and slots past j + 1 are zero in every
-                // row until later steps.
+                // row becomes pv * fr - fr * pv = 0, and slots past j are zero
+                // in every row until this step, so slot j + 1 is -fr * qc.
+                // Every lane computes and the pivot lane keeps its own values
+                // (selects, not branches, so the step stays one basic block);
+                // rows past n hold zeros and are never published.
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    if (!ok[h] || lane + 64 * h == pr)
-                        continue;
+                    const bool me = lane + 64 * h == pr;
                     const V fr = pn[h][j];
-                    pn[h][j] = 0;
 #pragma unroll
-                    for (int jj = j + 1; jj < P; ++jj)
-                        pn[h][jj] = ar.msub(pv, pn[h][jj], fr, qp[jj]);
-                    cc[h][0] = ar.mul(pv, cc[h][0]);
+                    for (int jj = j + 1; jj < P; ++jj) {
+                        const V t = ar.msub(pv, pn[h][jj], fr, qp[jj]);
+                        pn[h][jj] = me ? pn[h][jj] : t;
+                    }
+                    pn[h][j] = me ? fr : (V)0;
 #pragma unroll
-                    for (int s2 = 1; s2 <= j + 1; ++s2)
-                        cc[h][s2] = ar.msub(pv, cc[h][s2], fr, qc[s2]);
+                    for (int s2 = 1; s2 <= j; ++s2) {
+                        const V t = ar.msub(pv, cc[h][s2], fr, qc[s2]);
+                        cc[h][s2] = me ? cc[h][s2] : t;
+                    }
+                    const V tj = ar.msub(pv, (V)0, fr, qc[j + 1]);
+                    cc[h][j + 1] = me ? qc[j + 1] : tj;
+                    const V t0 = ar.mul(pv, cc[h][0]);
+                    cc[h][0] = me ? (V)0 : t0;
+                    isp[h] = isp[h] || me;
                 }
                 prs[j] = pr;
-                if (lane == 0) {
-                    prow[rank + nb] = pr;
-                    pivcol[rank + nb] = c0 + j;
-                    colpiv[c0 + j] = 1;
-                }
+                tk |= 1 << j;
                 ++nb;
             };
             static_assert(P >= 1 && P <= 4, "the panel steps below are written out for up to 4 columns");
@@ -594,6 +592,17 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 }
             }
             if (lane == 0) {
+                // pivot bookkeeping, read only after the elimination
+                int q = rank;
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    if (tk >> k & 1) {
+                        prow[q] = prs[k];
+                        pivcol[q] = c0 + k;
+                        colpiv[c0 + k] = 1;
+                        ++q;
+                    }
+                }
                 s_nb[buf] = nb;
 #pragma unroll
                 for (int k = 0; k < P; ++k)
