@@ -224,6 +224,25 @@ struct Mont64Arith {
         const Acc t = fma(prod(pv, x), fr, p - q);
         return redc(t.hi, t.lo);
     }
+    // K independent msubs sharing pv and fr, formed stage by stage so that
+    // their dependent multiplies interleave
+    template <int K>
+    __device__ void msub_n(V pv, const V (&x)[K], V fr, const V (&q)[K], V (&out)[K]) const
+    {
+        Acc t[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            t[i] = fma(prod(pv, x[i]), fr, p - q[i]);
+        uint64_t mm[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            mm[i] = t[i].lo * pinv;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint64_t u = t[i].hi + mulhi64(mm[i], p) + (t[i].lo != 0 ? 1 : 0);
+            out[i] = u >= p ? u - p : u;
+        }
+    }
     __device__ V sub(V a, V b) const { return a >= b ? a - b : a + (p - b); }
     __device__ V add(V a, V b) const
     {
@@ -271,6 +290,26 @@ struct Mont32Arith {
     __device__ V msub(V pv, V x, V fr, V q) const
     {
         return redc((uint64_t)pv * x + (uint64_t)fr * (p - q));
+    }
+    // K independent msubs sharing pv and fr, formed stage by stage so that
+    // their dependent multiplies interleave
+    template <int K>
+    __device__ void msub_n(V pv, const V (&x)[K], V fr, const V (&q)[K], V (&out)[K]) const
+    {
+        uint64_t t[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            t[i] = (uint64_t)pv * x[i] + (uint64_t)fr * (p - q[i]);
+        uint32_t mm[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            mm[i] = (uint32_t)t[i] * pinv;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t u = (uint32_t)((t[i] + (uint64_t)mm[i] * p) >> 32);
+            const uint32_t v = u - p;
+            out[i] = v < u ? v : u;
+        }
     }
     __device__ V sub(V a, V b) const
     {
@@ -321,6 +360,13 @@ struct BigArith {
     __device__ V msub(V pv, V x, V fr, V q) const
     {
         return addmod_any(mul(pv, x), mul(fr, q ? mc.p - q : 0), mc.p);
+    }
+    template <int K>
+    __device__ void msub_n(V pv, const V (&x)[K], V fr, const V (&q)[K], V (&out)[K]) const
+    {
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            out[i] = msub(pv, x[i], fr, q[i]);
     }
     __device__ V sub(V a, V b) const { return a >= b ? a - b : a + (mc.p - b); }
     __device__ V add(V a, V b) const { return addmod_any(a, b, mc.p); }
@@ -551,19 +597,30 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                 for (int h = 0; h < H; ++h) {
                     const bool me = lane + 64 * h == pr;
                     const V fr = pn[h][j];
+                    // the P products of this step: panel columns j + 1 .. P - 1,
+                    // slots 1 .. j, and slot j + 1 (from zero)
+                    V xs[P], qs[P], ts[P];
 #pragma unroll
                     for (int jj = j + 1; jj < P; ++jj) {
-                        const V t = ar.msub(pv, pn[h][jj], fr, qp[jj]);
-                        pn[h][jj] = me ? pn[h][jj] : t;
+                        xs[jj - j - 1] = pn[h][jj];
+                        qs[jj - j - 1] = qp[jj];
                     }
-                    pn[h][j] = me ? fr : (V)0;
 #pragma unroll
                     for (int s2 = 1; s2 <= j; ++s2) {
-                        const V t = ar.msub(pv, cc[h][s2], fr, qc[s2]);
-                        cc[h][s2] = me ? cc[h][s2] : t;
+                        xs[P - j - 2 + s2] = cc[h][s2];
+                        qs[P - j - 2 + s2] = qc[s2];
                     }
-                    const V tj = ar.msub(pv, (V)0, fr, qc[j + 1]);
-                    cc[h][j + 1] = me ? qc[j + 1] : tj;
+                    xs[P - 1] = 0;
+                    qs[P - 1] = qc[j + 1];
+                    ar.msub_n(pv, xs, fr, qs, ts);
+#pragma unroll
+                    for (int jj = j + 1; jj < P; ++jj)
+                        pn[h][jj] = me ? pn[h][jj] : ts[jj - j - 1];
+                    pn[h][j] = me ? fr : (V)0;
+#pragma unroll
+                    for (int s2 = 1; s2 <= j; ++s2)
+                        cc[h][s2] = me ? cc[h][s2] : ts[P - j - 2 + s2];
+                    cc[h][j + 1] = me ? qc[j + 1] : ts[P - 1];
                     const V t0 = ar.mul(pv, cc[h][0]);
                     cc[h][0] = me ? (V)0 : t0;
                     isp[h] = isp[h] || me;
