@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     int *perm = reinterpret_cast<int *>(Ck + 2 * kMaxN * (kPanel + 1));  // m
     int *plist = perm + m;                                           // m
     int *colpiv = plist + m;                                         // m (columns of At)
-    int *pos = colpiv + m;                                           // m
+    int *pos = colpiv + m;                                           // m (then: pivot row by column)
     int *pivcol = pos + m;                                           // kMaxN (by rank)
     int *prow = pivcol + kMaxN;                                      // kMaxN (by rank)
     int *ispiv = prow + kMaxN;                                       // kMaxN (rows of At)
@@ -657,6 +657,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                         prow[q] = prs[k];
                         pivcol[q] = c0 + k;
                         colpiv[c0 + k] = 1;
+                        pos[c0 + k] = prs[k];  // the column's pivot row
                         ++q;
                     }
                 }
@@ -668,17 +669,212 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         }
         __builtin_amdgcn_s_setprio(0);
     };
+    // Residuals of at most 32 columns (n <= 32 rows of At) leave the upper
+    // half of the composing wave idle in compose().  compose_half() puts the
+    // coefficient slots there: upper lane 32 + r holds slots 0, 1, 2 of row r
+    // (slots 0, 1 for P = 2) and lower lane r the row's panel values and the
+    // last slots, in registers whose lower halves pass from a panel column to
+    // a slot once that column is eliminated.  Each instruction then advances
+    // a panel value and a coefficient at once: three multiply-subtracts per
+    // pivot step instead of five (two instead of three for P = 2).
+    //   P = 4, registers Z A B C: lower lane r holds row r in panel columns
+    //   0..3, A becomes slot 3 before step 2 and B slot 4 before step 3;
+    //   upper lane 32 + r holds slots 0, 1, 2 in A, B, C.
+    //   P = 2, registers Z A B: lower lane r holds columns 0, 1 in Z, A and
+    //   slot 2 in B; upper lane 32 + r holds slots 0, 1 in A, B.
+    // Slot 0 takes pv * c0 as the multiply-subtract with q = 0 (pv c0 + fr p,
+    // the same residue); a register that starts a slot holds zero, so the
+    // slot's first value is -fr * q as in compose().
+    auto compose_half = [&](const int c0, const int buf) {
+        __builtin_amdgcn_s_setprio(2);
+        {
+            static_assert(P == 2 || P == 4, "compose_half is written out for panels of 2 and 4");
+            const bool hi = lane >= 32;
+            const int r = lane & 31;
+            const bool ok = r < n;
+            auto col = [&](const int j) -> V {
+                return !hi && ok && c0 + j < m ? At[r * ms + c0 + j] : (V)0;
+            };
+            V Z = col(0);
+            V A = hi ? ar.one() : col(1);
+            V B = P == 4 ? col(2) : (V)0;
+            V C = P == 4 ? col(3) : (V)0;
+            bool isp = !hi && ok && ispiv[r];
+            int prs[P] = {0};
+            int tk = 0, nb = 0;
+            // every lane gets the value of x in lane (lane & 31)
+            auto low = [](const V x) -> V {
+                if constexpr (sizeof(V) == 4) {
+                    return __builtin_amdgcn_permlane32_swap(x, x, false, false)[0];
+                } else {
+                    const uint32_t l = (uint32_t)x, h = (uint32_t)(x >> 32);
+                    const uint32_t l2 = __builtin_amdgcn_permlane32_swap(l, l, false, false)[0];
+                    const uint32_t h2 = __builtin_amdgcn_permlane32_swap(h, h, false, false)[0];
+                    return ((uint64_t)h2 << 32) | l2;
+                }
+            };
+            auto step = [&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                if constexpr (P == 4 && j == 2)
+                    A = hi ? A : (V)0;  // column 1 is done: lower A is slot 3
+                if constexpr (P == 4 && j == 3)
+                    B = hi ? B : (V)0;  // column 2 is done: lower B is slot 4
+                const V xj = j == 0 ? Z : j == 1 ? A : j == 2 ? B : C;
+                const bool open = c0 + j < m && rank + nb < n;
+                const unsigned long long b = __ballot(open && !hi && ok && !isp && xj != 0);
+                if (!b)
+                    return;  // no pivot in this column
+                const int pr = __ffsll(b) - 1;  // a lower lane
+                const bool me = r == pr;
+                const V pv = lane_value(xj, pr);
+                const V fr = low(xj);
+                const V Q = lane_value(A, 32 + pr);  // the pivot row's slot 0
+                constexpr int G = P == 4 ? 3 : 2;
+                V xs[G], qs[G], ts[G], kp[G];  // operands, and what the pivot row keeps
+                const V zero = 0;
+                if constexpr (P == 4) {
+                    xs[0] = A;
+                    xs[1] = B;
+                    xs[2] = C;
+                    if constexpr (j == 0) {
+                        const V p1 = lane_value(A, pr), p2 = lane_value(B, pr), p3 = lane_value(C, pr);
+                        qs[0] = hi ? zero : p1;
+                        qs[1] = hi ? Q : p2;
+                        qs[2] = hi ? zero : p3;
+                        kp[0] = hi ? zero : A;
+                        kp[1] = hi ? Q : B;
+                        kp[2] = C;
+                    } else if constexpr (j == 1) {
+                        const V p2 = lane_value(B, pr), p3 = lane_value(C, pr), q1 = lane_value(B, 32 + pr);
+                        qs[0] = zero;
+                        qs[1] = hi ? q1 : p2;
+                        qs[2] = hi ? Q : p3;
+                        kp[0] = hi ? zero : A;
+                        kp[1] = B;
+                        kp[2] = hi ? Q : C;
+                    } else if constexpr (j == 2) {
+                        const V p3 = lane_value(C, pr), q1 = lane_value(B, 32 + pr), q2 = lane_value(C, 32 + pr);
+                        qs[0] = hi ? zero : Q;
+                        qs[1] = hi ? q1 : zero;
+                        qs[2] = hi ? q2 : p3;
+                        kp[0] = hi ? zero : Q;
+                        kp[1] = B;
+                        kp[2] = C;
+                    } else {
+                        const V q3 = lane_value(A, pr), q1 = lane_value(B, 32 + pr), q2 = lane_value(C, 32 + pr);
+                        qs[0] = hi ? zero : q3;
+                        qs[1] = hi ? q1 : Q;
+                        qs[2] = hi ? q2 : zero;
+                        kp[0] = hi ? zero : A;
+                        kp[1] = hi ? B : Q;
+                        kp[2] = C;
+                    }
+                } else {
+                    xs[0] = A;
+                    xs[1] = B;
+                    if constexpr (j == 0) {
+                        const V p1 = lane_value(A, pr);
+                        qs[0] = hi ? zero : p1;
+                        qs[1] = hi ? Q : zero;
+                        kp[0] = hi ? zero : A;
+                        kp[1] = hi ? Q : B;
+                    } else {
+                        const V q1 = lane_value(B, 32 + pr);
+                        qs[0] = zero;
+                        qs[1] = hi ? q1 : Q;
+                        kp[0] = hi ? zero : A;
+                        kp[1] = hi ? B : Q;
+                    }
+                }
+                ar.msub_n(pv, xs, fr, qs, ts);
+                A = me ? kp[0] : ts[0];
+                B = me ? kp[1] : ts[1];
+                if constexpr (P == 4)
+                    C = me ? kp[2] : ts[2];
+                isp = isp || me;  // read in the lower half only
+                prs[j] = pr;
+                tk |= 1 << j;
+                ++nb;
+            };
+            step(std::integral_constant<int, 0>{});
+            step(std::integral_constant<int, 1>{});
+            if constexpr (P == 4) {
+                step(std::integral_constant<int, 2>{});
+                step(std::integral_constant<int, 3>{});
+            }
+            // publish: coefficients, pivot flags, pivot rows
+            if (ok) {
+                V *ck = Ck + (buf * kMaxN + r) * (P + 1);
+                if (hi) {
+                    ck[0] = A;
+                    ck[1] = B;
+                    if constexpr (P == 4)
+                        ck[2] = C;
+                } else {
+                    if constexpr (P == 4) {
+                        ck[3] = A;
+                        ck[4] = B;
+                    } else {
+                        ck[2] = B;
+                    }
+                    ispiv[r] = isp ? 1 : 0;
+                }
+            }
+            if (lane == 0) {
+                // pivot bookkeeping, read only after the elimination
+                int q = rank;
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    if (tk >> k & 1) {
+                        prow[q] = prs[k];
+                        pivcol[q] = c0 + k;
+                        colpiv[c0 + k] = 1;
+                        pos[c0 + k] = prs[k];  // the column's pivot row
+                        ++q;
+                    }
+                }
+                s_nb[buf] = nb;
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                    s_prs[buf][k] = prs[k];
+            }
+        }
+        __builtin_amdgcn_s_setprio(0);
+    };
+    // the split composition needs one row per lane and the Montgomery forms
+    constexpr bool kHalf = H == 1 && !std::is_same<Arith, BigArith>::value;
+    const bool half = kHalf && n <= 32;
+    auto compose_panel = [&](const int c0, const int buf) {
+        if constexpr (kHalf) {
+            if (half) {
+                compose_half(c0, buf);
+                return;
+            }
+        }
+        compose(c0, buf);
+    };
     // apply the combination in buffer buf to this wave's columns: the pivot
     // rows' values are read before any row is written.  Every term is
     // formed: a panel column without a pivot has row 0 in s_prs and zero
     // coefficients in every row, so its term is zero, and issuing all loads
     // before the products (no branch between them) keeps the LDS round
     // trips of a panel to three instead of one per term.
+    // A pivot column of an earlier panel (below cdone) is zero but in its
+    // pivot row, and no later pivot row has a nonzero there: the combination
+    // only multiplies that row's value by the row's own coefficient.
     const int acl = lane & (cw - 1), ag = lane / cw, aG = 64 / cw;  // cw: a power of two
     const int acg = wave * cw + acl;
-    auto apply = [&](const int buf) {
+    auto apply = [&](const int buf, const int cdone) {
         if (acg >= m)
             return;
+        if (acg < cdone && colpiv[acg]) {
+            const int r = pos[acg];
+            if (ag == (r & (aG - 1))) {
+                V *x = At + r * ms + acg;
+                *x = ar.mul(Ck[(buf * kMaxN + r) * (P + 1)], *x);
+            }
+            return;
+        }
         int prq[P];
 #pragma unroll
         for (int q = 0; q < P; ++q)
@@ -703,19 +899,19 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
         }
     };
     if (wave == 0)
-        compose(0, 0);
+        compose_panel(0, 0);
     __syncthreads();
     for (int k = 0;; ++k) {
         const int buf = k & 1;
         const int nb = s_nb[buf];
         if (nb > 0)
-            apply(buf);
+            apply(buf, k * P);
         rank += nb;
         const int c1 = (k + 1) * P;
         if (c1 >= m || rank >= n)
             break;
         if (wave == c1 / cw)
-            compose(c1, buf ^ 1);
+            compose_panel(c1, buf ^ 1);
         __syncthreads();
     }
     __syncthreads();
