@@ -194,15 +194,16 @@ def test_pmbasis_device_resident(pml, m, n, lenF, order, p):
 
 SWEEP_DIMS = [(1, 1), (2, 1), (31, 17), (32, 32), (33, 20), (40, 33), (63, 63), (64, 64),
               (65, 40), (66, 65), (96, 70), (127, 100), (128, 1), (128, 64), (128, 65),
-              (128, 128), (5, 100), (17, 128)]
+              (128, 128), (5, 100), (17, 128), (100, 32), (128, 31), (256, 32)]
 
 
 @pytest.mark.parametrize("p", [P30, P60])
 @pytest.mark.parametrize("m,n", SWEEP_DIMS)
 def test_pmbasis_dimension_sweep(pml, m, n, p):
     # the step kernel's column blocks change at m = 32 and m = 64, its rows per
-    # lane at n = 64, its panel width with the word size; short orders keep
-    # the oracle quick, and include residuals of full and deficient rank
+    # lane at n = 64, its composition (coefficients on the wave's upper half)
+    # at n = 32, its panel width with the word size; short orders keep the
+    # oracle quick, and include residuals of full and deficient rank
     rng = np.random.default_rng(m * 1009 + n * 7 + (p & 0xff))
     order = int(rng.integers(3, 9))
     F = make(rng, "uniform" if (m + n) % 2 else "rkdef", m, n, order, p)
