@@ -536,8 +536,10 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     int cw = P;
     while (16 * cw < m)
         cw *= 2;
-    // compose the panel of columns c0 .. c0 + P - 1 into buffer buf
-    auto compose = [&](const int c0, const int buf) {
+    // compose the panel of columns c0 .. c0 + P - 1 into buffer buf (the
+    // lambdas are forced inline: called from two places, compose was left as
+    // a call in the two-row kernels, its captures and arrays on the stack)
+    auto compose = [&](const int c0, const int buf) __attribute__((always_inline)) {
         __builtin_amdgcn_s_setprio(2);
         {
             // rows lane + 64 h (h < H) of the panel, their pivot flags and the
@@ -563,7 +565,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
             // one pivot step on panel column j (a compile-time index: the
             // convergent ballots and lane reads keep the compiler from unrolling
             // a loop around them, and the arrays must stay in registers)
-            auto step = [&](auto jc) {
+            auto step = [&](auto jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(jc)::value;
                 // columns past m, or a full rank already reached, take no pivot
                 const bool open = c0 + j < m && rank + nb < n;
@@ -685,7 +687,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // Slot 0 takes pv * c0 as the multiply-subtract with q = 0 (pv c0 + fr p,
     // the same residue); a register that starts a slot holds zero, so the
     // slot's first value is -fr * q as in compose().
-    auto compose_half = [&](const int c0, const int buf) {
+    auto compose_half = [&](const int c0, const int buf) __attribute__((always_inline)) {
         __builtin_amdgcn_s_setprio(2);
         {
             static_assert(P == 2 || P == 4, "compose_half is written out for panels of 2 and 4");
@@ -713,7 +715,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
                     return ((uint64_t)h2 << 32) | l2;
                 }
             };
-            auto step = [&](auto jc) {
+            auto step = [&](auto jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(jc)::value;
                 if constexpr (P == 4 && j == 2)
                     A = hi ? A : (V)0;  // column 1 is done: lower A is slot 3
@@ -844,7 +846,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // the split composition needs one row per lane and the Montgomery forms
     constexpr bool kHalf = H == 1 && !std::is_same<Arith, BigArith>::value;
     const bool half = kHalf && n <= 32;
-    auto compose_panel = [&](const int c0, const int buf) {
+    auto compose_panel = [&](const int c0, const int buf) __attribute__((always_inline)) {
         if constexpr (kHalf) {
             if (half) {
                 compose_half(c0, buf);
@@ -864,7 +866,7 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
     // only multiplies that row's value by the row's own coefficient.
     const int acl = lane & (cw - 1), ag = lane / cw, aG = 64 / cw;  // cw: a power of two
     const int acg = wave * cw + acl;
-    auto apply = [&](const int buf, const int cdone) {
+    auto apply = [&](const int buf, const int cdone) __attribute__((always_inline)) {
         if (acg >= m)
             return;
         if (acg < cdone && colpiv[acg]) {
