@@ -430,9 +430,11 @@ __global__ void __launch_bounds__(kStepThreads) mbasis_step_kernel(LeafArgs a, i
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     using V = typename Arith::V;
-    // panel width: 4 columns for 32-bit words, 2 for 64-bit ones, whose
-    // dearer products make a shorter composition pay for the extra barriers
-    constexpr int P = sizeof(V) == 8 ? 2 : kPanel;
+    // panel width: 4 columns for 32-bit words, 2 for 64-bit ones in the
+    // one-row kernels, whose dearer products make a shorter composition pay
+    // for the extra barriers; the two-row kernels (more than 64 residual
+    // columns) are bound by the apply, which wider panels make shorter
+    constexpr int P = sizeof(V) == 8 && H == 1 ? 2 : kPanel;
     static_assert(P <= kPanel, "LDS is sized for kPanel");
     const int m = a.m, n = a.n;
     const Arith ar(a.mc);

@@ -61,7 +61,7 @@ def run_mul(a):
 
 
 def run_pmbasis(a):
-    m, n, order = 64, 32, 2048
+    m, n, order = (int(x) for x in a.shape.split("x"))
     p = a.p
     ctx = Context(p)
     ctx.set_leaf_order(a.leaf_order)
@@ -91,7 +91,7 @@ def run_pmbasis(a):
         not D[:, :, Lh:].any()
     # residual P*F mod x^order on the device
     R = ctx.mulmid(P, F, 0, order)
-    out = {"workload": "pmbasis 64x32 order 2048 p=%d shift 0" % p, "seconds_per_basis": round(dt, 5),
+    out = {"workload": "pmbasis %dx%d order %d p=%d shift 0" % (m, n, order, p), "seconds_per_basis": round(dt, 5),
            "bases_per_s": round(1.0 / dt, 3), "seconds_per_basis_device_resident": round(dt_dev, 5),
            "device_resident_identical": bool(same), "basis_length": int(P.shape[2]),
            "shift_out_sum": int(sh.sum()), "residual_zero": bool(not R.any()), "reps": a.reps,
@@ -237,6 +237,7 @@ def main(argv=None):
     p.add_argument("--p", type=int, default=P30, help="modulus (default 998244353)")
     p.add_argument("--leaf-order", type=int, default=0,
                    help="M-Basis/PM-Basis crossover: 0 measures it (default), 1..256 fixes it")
+    p.add_argument("--shape", default="64x32x2048", help="m x n x order of F (default 64x32x2048)")
     mu = sub.add_parser("mul")
     mu.add_argument("--reps", type=int, default=20)
     mu.add_argument("--p", type=int, default=P60, help="modulus (default 2^60-3*2^24+1)")
