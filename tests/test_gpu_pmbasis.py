@@ -294,3 +294,30 @@ def test_wide_basis_beyond_lds_refused(pml):
     with pytest.raises(pml.PmlgError) as ei:
         ctx.pmbasis(O.rand_uniform((257, 4, 8), P60, 1), 8, np.zeros(257, dtype=np.int64))
     assert ei.value.status == 1
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_pmbasis_randomized_sweep(pml, seed):
+    # Randomized like the reference's t-pmbasis.c:70-109 (random prime,
+    # dimensions, order and shifts per iteration): 150 seeded draws over
+    # FFT and non-FFT primes (p = 2 and a 59-bit prime included), matrix
+    # families and shift families, M-Basis and PM-Basis, each identical to
+    # the oracle's basis and shift
+    rng = np.random.default_rng(seed)
+    primes = [P30, P60, 97, 7340033, 3, 576460752303423619, 2]
+    small = (2, 3, 97, 576460752303423619)
+    for it in range(150):
+        p = primes[rng.integers(len(primes))]
+        m = int(rng.integers(1, 90))
+        n = int(rng.integers(1, 40 if rng.random() < 0.7 else 100))
+        order = int(rng.integers(1, 24 if p in small else 48))
+        kind = ["uniform", "sparse", "rkdef", "cdeg", "zero"][rng.integers(5)]
+        F = make(rng, kind, m, n, order, p)
+        sname = sorted(SHIFTS)[rng.integers(len(SHIFTS))]
+        shift = SHIFTS[sname](m, order).astype(np.int64) + rng.integers(-2, 3, m)
+        fn = "mbasis" if rng.random() < 0.3 else "pmbasis"
+        P_ref, s_ref = getattr(O, fn)(F, order, shift, p)
+        P_dev, s_dev = getattr(ctx_for(pml, p), fn)(F, order, shift)
+        case = (it, fn, p, m, n, order, kind, sname)
+        assert np.array_equal(O.normalise(P_dev), O.normalise(P_ref)), case
+        assert np.array_equal(s_dev, s_ref), case
