@@ -198,3 +198,35 @@ def test_mfma_inner_dimension_bound(pml):
     assert np.array_equal(ctx.mul(A, B, method="auto"), O.polmat_mul(A, B, p))
     with pytest.raises(pml.PmlgError):
         ctx.mul(A, B, method="mfma")
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_mul_randomized_sweep(pml, seed):
+    # Randomized like the reference's t-mul_geometric.c:24-80 (a random
+    # prime, dimensions and lengths per iteration): 200 seeded products or
+    # middle products over FFT primes, non-FFT primes (the CRT path) and
+    # moduli up to 2^64, with each form, identical to the oracle's
+    rng = np.random.default_rng(seed)
+    primes = [P30, P60, 7340033, 469762049, 97, 674830283164631843, 2**64 - 59, 2**63 + 29,
+              1000000007, 2]
+    for it in range(200):
+        p = int(primes[rng.integers(len(primes))])
+        fft = p in (P30, P60, 7340033, 469762049)
+        m, k, n = (int(x) for x in rng.integers(1, 20 if fft else 7, 3))
+        top = 3000 if fft else 200
+        la, lb = int(rng.integers(1, top)), int(rng.integers(1, top))
+        if rng.random() < 0.3:
+            la = int(rng.integers(1, 20))
+        A = O.rand_uniform((m, k, la), p, int(rng.integers(1 << 30)))
+        B = O.rand_uniform((k, n, lb), p, int(rng.integers(1 << 30)))
+        method = ["auto", "plain", "mfma"][rng.integers(3)]
+        ctx = ctx_for(pml, p)
+        case = (it, p, m, k, n, la, lb, method)
+        if rng.random() < 0.7:
+            assert np.array_equal(ctx.mul(A, B, method=method), O.polmat_mul(A, B, p)), case
+        else:
+            lc = la + lb - 1
+            nlo = int(rng.integers(0, lc))
+            nhi = int(rng.integers(nlo + 1, lc + 1))
+            assert np.array_equal(ctx.mulmid(A, B, nlo, nhi, method=method),
+                                  O.polmat_mulmid(A, B, nlo, nhi, p)), case
