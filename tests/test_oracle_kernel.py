@@ -16,11 +16,19 @@ fixes, not to a retyping of its own steps:
 - is_kernel rejects planted mistakes (a dropped row, a perturbed coefficient,
   two rows swapped).
 """
+import zlib
+
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from oracle import kernel as KO
+
+
+def _seed(*names):
+    # stable across processes (str hashes are salted per process)
+    return zlib.crc32("/".join(names).encode())
+
 
 P60 = 1152921504556515329
 P30 = 998244353
@@ -112,7 +120,7 @@ def test_generic_degree_sum(m, n, d):
 @pytest.mark.parametrize("shift_name", sorted(SHIFTS))
 @pytest.mark.parametrize("kind", ["uniform", "zero", "sparse", "rdeg", "cdeg", "rkdef"])
 def test_two_algorithms_agree(shift_name, kind):
-    rng = np.random.default_rng(hash((shift_name, kind)) % (1 << 32))
+    rng = np.random.default_rng(_seed(shift_name, kind))
     for it in range(3):
         p = (P30, P60)[it % 2]
         m, n = int(rng.integers(2, 8)), int(rng.integers(1, 6))
